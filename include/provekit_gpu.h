@@ -1,0 +1,137 @@
+/*
+ * provekit_gpu — C ABI of the MI355X (gfx950) implementation of ProveKit's
+ * WHIR commitment hot path: Reed-Solomon encoding by NTT over BN254 Fr,
+ * Skyscraper / Poseidon2 Merkle hashing, and the R1CS sparse products and
+ * sumcheck rounds.
+ *
+ * Conventions
+ *  - Field elements are ark-ff `ark_bn254::Fr` values as laid out in memory:
+ *    4 little-endian u64 limbs of x * 2^256 mod p (Montgomery form), fully
+ *    reduced.  A `uint64_t*` "Fr array" of n elements holds 4n limbs.
+ *  - Skyscraper byte buffers are the reference's `[u8]` views: each 32-byte
+ *    word is a little-endian 256-bit integer (any value; reduced mod p).
+ *  - Every function returns a pkgpu_status; on failure the calling thread's
+ *    message is available from pkgpu_last_error() (the FFI pattern of
+ *    tooling/provekit-ffi/src/ffi.rs:29-87).  Nothing unwinds across the ABI.
+ *  - "host" entry points take caller-owned host buffers (the reference's
+ *    slices) and return when the result is in them.  "_device" entry points
+ *    take device pointers (from pkgpu_malloc or any HIP allocation on the
+ *    current device) and a hipStream_t passed as void* (NULL: the calling
+ *    thread's stream); they return after enqueueing.
+ *  - All entry points use the calling thread's current HIP device
+ *    (pkgpu_init selects it).  The library is thread-safe.
+ */
+#ifndef PROVEKIT_GPU_H
+#define PROVEKIT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PKGPU_ABI_VERSION 1
+
+typedef enum {
+  PKGPU_OK = 0,
+  PKGPU_ERR_ARG = 1,      /* null pointer, length mismatch, bad option */
+  PKGPU_ERR_NOT_POW2 = 2, /* a size must be a power of two */
+  PKGPU_ERR_SIZE = 3,     /* divisibility or range violation (e.g. order above 2^28) */
+  PKGPU_ERR_OOM = 4,      /* device allocation failed */
+  PKGPU_ERR_DEVICE = 5,   /* HIP runtime error */
+  PKGPU_ERR_BUDGET = 6,   /* serving admission refused: HBM budget */
+  PKGPU_ERR_STATE = 7,    /* unknown handle or handle in the wrong state */
+  PKGPU_ERR_INTERNAL = 8  /* a self-check failed */
+} pkgpu_status;
+
+/* ------------------------------------------------------------ library */
+/* Select device `device` for the calling thread and initialise it.
+ * Replaces provekit_common::register_ntt (provekit/common/src/lib.rs:44-63)
+ * as the one-time set-up call; idempotent. */
+int pkgpu_init(int device);
+int pkgpu_abi_version(void);
+/* Message of the calling thread's last failure ("" if none). */
+const char* pkgpu_last_error(void);
+int pkgpu_device_synchronize(void);
+
+/* Device memory helpers for callers without their own HIP allocator. */
+int pkgpu_malloc(void** ptr, size_t bytes);
+int pkgpu_free(void* ptr);
+int pkgpu_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int pkgpu_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int pkgpu_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------ ntt crate */
+/* ntt::ntt_nr(values, codeword_size, num_groups) (ntt/src/ntt.rs:105-108):
+ * in-place NTT, natural -> bit-reversed order, of len / codeword_size
+ * interleaved polynomials, starting at stage log2(num_groups).
+ * Errors as the reference's asserts (ntt.rs:164-169): len % num_groups,
+ * non-power-of-two codeword_size. */
+int pkgpu_ntt_nr(uint64_t* values, size_t len, size_t codeword_size, size_t num_groups);
+int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size_t num_groups,
+                        void* stream);
+/* ntt::intt_rn(values) (ntt/src/ntt.rs:326-330): inverse, bit-reversed ->
+ * natural order. */
+int pkgpu_intt_rn(uint64_t* values, size_t len);
+int pkgpu_intt_rn_device(uint64_t* values, size_t len, void* stream);
+
+/* ------------------------------------------------------------ RSFr plugin */
+/* ReedSolomon<Fr> for RSFr (provekit/common/src/ntt.rs:12-105). */
+/* next_order: PKGPU_ERR_SIZE where the reference returns None (> 2^28). */
+int pkgpu_rs_next_order(size_t size, size_t* order);
+/* generator(codeword_length): the primitive codeword_length-th root of unity. */
+int pkgpu_rs_generator(size_t codeword_length, uint64_t out[4]);
+/* evaluation_points: out[t] = generator^{bitrev(indices[t])}. */
+int pkgpu_rs_evaluation_points(size_t masked_message_length, size_t codeword_length,
+                               const size_t* indices, size_t n, uint64_t* out);
+/* interleaved_encode(messages, masks, codeword_length): messages[r] points at
+ * message_length elements; out receives num_messages * codeword_length. */
+int pkgpu_rs_interleaved_encode(const uint64_t* const* messages, size_t num_messages,
+                                size_t message_length, const uint64_t* masks, size_t masks_len,
+                                size_t codeword_length, uint64_t* out);
+/* Device form: messages is one row-major num_messages x message_length
+ * array.  coset != 0 evaluates on the coset 5*H (coefficient j scaled by 5^j;
+ * 5 = multiplicative generator of Fr) instead of H. */
+int pkgpu_rs_interleaved_encode_device(const uint64_t* messages, size_t num_messages,
+                                       size_t message_length, const uint64_t* masks,
+                                       size_t masks_len, size_t codeword_length, int coset,
+                                       uint64_t* out, void* stream);
+
+/* ------------------------------------------------------------ Skyscraper */
+/* skyscraper::simple::compress(l, r) (skyscraper/core/src/simple.rs:12-14). */
+int pkgpu_skyscraper_compress(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]);
+/* skyscraper::simple::compress_many(messages, hashes) (simple.rs:3-9):
+ * messages_len == 2 * hashes_len, multiples of 64 and 32 bytes. */
+int pkgpu_skyscraper_compress_many(const uint8_t* messages, size_t messages_len,
+                                   uint8_t* hashes, size_t hashes_len);
+/* SkyscraperHashEngine::hash_many(size, input, output)
+ * (provekit/common/src/skyscraper/whir.rs:42-100): count = output digests,
+ * input_len must equal size * count, size a positive multiple of 32. */
+int pkgpu_skyscraper_hash_many(size_t size, const uint8_t* input, size_t input_len,
+                               uint8_t* output, size_t count);
+int pkgpu_skyscraper_hash_many_device(size_t size, const void* input, size_t count, void* output,
+                                      void* stream);
+/* Binary Merkle tree of n (power of two) 32-byte leaf digests with
+ * compress(left, right); nodes receives 2n-1 digests level by level, root
+ * last (nodes may equal leaves). */
+int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, void* stream);
+
+/* ------------------------------------------------------------ commit */
+/* WHIR-style commitment of one polynomial of n coefficients (n, blowup powers
+ * of two): RS-encode to n*blowup elements (RSFr order; coset != 0: on 5*H),
+ * hash leaves of leaf_elems consecutive codeword elements by left fold
+ * (hash_many with size 32*leaf_elems), binary Skyscraper tree to the root.
+ * This is the work of WhirR1CSScheme::commit -> whir commit
+ * (provekit/prover/src/whir_r1cs.rs:82-140) with the leaf layout fixed here.
+ * Host form: codeword (n*blowup Fr) and nodes (2*leaves-1 digests) may be
+ * NULL; root receives 32 bytes. */
+int pkgpu_commit(const uint64_t* coeffs, size_t n, size_t blowup, int coset, size_t leaf_elems,
+                 uint64_t* codeword, uint8_t* nodes, uint8_t root[32]);
+int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int coset,
+                        size_t leaf_elems, uint64_t* codeword, void* nodes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PROVEKIT_GPU_H */

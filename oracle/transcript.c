@@ -1,0 +1,144 @@
+/* ORACLE — test infrastructure only.
+ *
+ * The Fiat-Shamir transcript is spongefish in the reference (git rev fcc277f,
+ * not vendored, so its challenge bytes cannot be reproduced).  This project
+ * defines its own transcript from the pinned Skyscraper compression, and this
+ * file restates it independently of the product for the parity tests:
+ *   init(seed):   s = compress([seed, 0, 0, 0], [TAG0, TAG1, 0, 0])
+ *   absorb(x):    s = compress(s, x)          (x: 4 Montgomery limbs of Fr)
+ *   squeeze():    s = compress(s, [0,0,0,0]); the challenge is the Fr whose
+ *                 Montgomery limbs are s (s < p because compress reduces)
+ * and the ZK sumcheck driver run_zk_sumcheck_prover
+ * (provekit/prover/src/whir_r1cs.rs:519-621) with
+ * compute_blinding_coefficients_for_round (:427-492) and
+ * sum_over_hypercube (:494-501).
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define TAG0 0x70726f76656b6974ULL /* "provekit" */
+#define TAG1 0x73756d636865636bULL /* "sumcheck" */
+
+typedef struct {
+  uint64_t s[4];
+} or_transcript;
+
+static void tr_init(or_transcript* t, uint64_t seed) {
+  uint64_t l[4] = {seed, 0, 0, 0}, r[4] = {TAG0, TAG1, 0, 0};
+  or_sky_compress(l, r, t->s);
+}
+static void tr_absorb(or_transcript* t, ofr x) { or_sky_compress(t->s, x.l, t->s); }
+static ofr tr_squeeze(or_transcript* t) {
+  static const uint64_t z[4] = {0, 0, 0, 0};
+  or_sky_compress(t->s, z, t->s);
+  ofr c;
+  memcpy(c.l, t->s, 32);
+  return c;
+}
+
+/* compute_blinding_coefficients_for_round (whir_r1cs.rs:427-492) */
+static void blinding_coeffs(const ofr* g, size_t n, size_t compute_for, const ofr* alphas,
+                            ofr out[4]) {
+  int all_fixed = 0;
+  if (compute_for == n) {
+    all_fixed = 1;
+    compute_for = n - 1;
+  }
+  ofr zero = {{0}};
+  ofr prefix = zero;
+  for (size_t i = 0; i < compute_for; i++) prefix = ofr_add(prefix, or_eval_cubic(g + 4 * i, alphas[i]));
+  ofr suffix = zero;
+  for (size_t i = compute_for + 1; i < n; i++)
+    suffix = ofr_add(suffix, ofr_add(or_eval_cubic(g + 4 * i, zero), or_eval_cubic(g + 4 * i, OFR_ONE)));
+  ofr pm = OFR_ONE;
+  for (size_t i = 0; i < n - 1 - compute_for; i++) pm = ofr_add(pm, pm);
+  ofr two = ofr_add(OFR_ONE, OFR_ONE);
+  ofr sm = ofr_mul(pm, ofr_inv(two));
+  ofr other = ofr_add(ofr_mul(pm, prefix), ofr_mul(sm, suffix));
+  const ofr* cur = g + 4 * compute_for;
+  ofr c[4] = {ofr_add(ofr_mul(pm, cur[0]), other), ofr_mul(pm, cur[1]), ofr_mul(pm, cur[2]),
+              ofr_mul(pm, cur[3])};
+  if (all_fixed) {
+    out[0] = or_eval_cubic(c, alphas[compute_for]);
+    out[1] = out[2] = out[3] = zero;
+    return;
+  }
+  for (int k = 0; k < 4; k++) out[k] = c[k];
+}
+
+/* sum_over_hypercube (whir_r1cs.rs:494-501) */
+static ofr sum_over_hypercube(const ofr* g, size_t n) {
+  ofr c[4];
+  blinding_coeffs(g, n, 0, NULL, c);
+  ofr zero = {{0}};
+  return ofr_add(or_eval_cubic(c, zero), or_eval_cubic(c, OFR_ONE));
+}
+
+/* Full sumcheck over eq(r, x) * (a(x) b(x) - c(x)), size = 2^m.
+ * blinding != NULL: reference mode (whir_r1cs.rs:519-621): absorb sum_g,
+ *   squeeze rho, claimed sum rho * sum_g, blinding g (m x 4) mixed in.
+ * blinding == NULL: plain mode: rho = 0, the claimed sum is computed as
+ *   sum_x eq(r,x)(a b - c) and absorbed in place of sum_g.
+ * a, b, c are consumed (folded in place).  coeffs: m x 4, alphas: m,
+ * r_out: m (the eq point), final_claim: the last cubic at alpha_{m-1}.
+ * Returns 0, or -1 if the round identity c0+c0+c1+c2+c3 = S fails. */
+int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uint64_t seed,
+                      ofr* coeffs, ofr* alphas, ofr* r_out, ofr* final_claim) {
+  size_t size = (size_t)1 << m;
+  if (size < 2) size = 2;
+  or_transcript tr;
+  tr_init(&tr, seed);
+  for (size_t i = 0; i < m; i++) r_out[i] = tr_squeeze(&tr);
+  ofr* eq = (ofr*)calloc(size, sizeof(ofr));
+  or_eq_evals(r_out, m, (size_t)1 << m, eq);
+  ofr zero = {{0}};
+  ofr rho = zero, S;
+  if (blinding) {
+    ofr sg = sum_over_hypercube(blinding, m);
+    tr_absorb(&tr, sg);
+    rho = tr_squeeze(&tr);
+    S = ofr_mul(rho, sg);
+  } else {
+    S = zero;
+    for (size_t i = 0; i < size; i++)
+      S = ofr_add(S, ofr_mul(eq[i], ofr_sub(ofr_mul(a[i], b[i]), c[i])));
+    tr_absorb(&tr, S);
+  }
+  ofr half = ofr_inv(ofr_add(OFR_ONE, OFR_ONE));
+  int ok = 0;
+  size_t cur = size;
+  for (size_t idx = 0; idx < m; idx++) {
+    ofr h[3];
+    const ofr* fold = idx ? &alphas[idx - 1] : NULL;
+    or_sumcheck_r1cs_round(a, b, c, eq, cur, fold, h);
+    if (fold) cur /= 2;
+    ofr g[4] = {zero, zero, zero, zero};
+    if (blinding) blinding_coeffs(blinding, m, idx, alphas, g);
+    ofr* co = coeffs + 4 * idx;
+    co[0] = ofr_add(h[0], ofr_mul(rho, g[0]));
+    ofr g_m1 = ofr_sub(ofr_add(ofr_sub(g[0], g[1]), g[2]), g[3]);
+    ofr at_m1 = ofr_add(h[1], ofr_mul(rho, g_m1));
+    ofr three_c0 = ofr_add(ofr_add(co[0], co[0]), co[0]);
+    co[2] = ofr_mul(half, ofr_sub(ofr_add(S, at_m1), three_c0));
+    co[3] = ofr_add(h[2], ofr_mul(rho, g[3]));
+    co[1] = ofr_sub(ofr_sub(ofr_sub(ofr_sub(S, co[0]), co[0]), co[3]), co[2]);
+    ofr chk = ofr_add(ofr_add(ofr_add(ofr_add(co[0], co[0]), co[1]), co[2]), co[3]);
+    if (!ofr_eq(chk, S)) ok = -1;
+    for (int k = 0; k < 4; k++) tr_absorb(&tr, co[k]);
+    alphas[idx] = tr_squeeze(&tr);
+    S = or_eval_cubic(co, alphas[idx]);
+  }
+  *final_claim = S;
+  free(eq);
+  return ok;
+}
+
+/* transcript primitives exported for tests */
+void or_transcript_run(uint64_t seed, const ofr* absorbs, size_t n_abs, ofr* squeezes,
+                       size_t n_sq) {
+  or_transcript t;
+  tr_init(&t, seed);
+  for (size_t i = 0; i < n_abs; i++) tr_absorb(&t, absorbs[i]);
+  for (size_t i = 0; i < n_sq; i++) squeezes[i] = tr_squeeze(&t);
+}

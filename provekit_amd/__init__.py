@@ -1,0 +1,319 @@
+"""provekit_amd — MI355X (gfx950) implementation of ProveKit's WHIR commitment
+hot path, exposed through the C ABI in include/provekit_gpu.h.
+
+This Python module is a thin ctypes binding with the reference's names
+(``ntt_nr``, ``intt_rn``, ``RSFr``, ``skyscraper.compress_many``,
+``SkyscraperHashEngine``, ...), used by the tests and the benchmark.  Every
+call runs the HIP kernels in ``libprovekit_gpu.so``; there is no CPU path.  If
+the library is missing, calls raise instead of falling back.
+
+Field elements are numpy ``uint64`` arrays of shape (n, 4): the Montgomery
+limbs of ark-ff's ``ark_bn254::Fr``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprovekit_gpu.so")
+
+# status codes (include/provekit_gpu.h)
+OK, ERR_ARG, ERR_NOT_POW2, ERR_SIZE, ERR_OOM, ERR_DEVICE, ERR_BUDGET, ERR_STATE, ERR_INTERNAL = range(9)
+
+
+class ProvekitGpuError(RuntimeError):
+    """A pkgpu_* call failed; ``code`` is the pkgpu_status, the message is the
+    library's (the reference's panic text where the reference panics)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pkgpu status {code}] {msg}")
+        self.code = code
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+# name -> argtypes; every function returns int (pkgpu_status) except the two below
+SIGNATURES = {
+    "pkgpu_init": [_i],
+    "pkgpu_device_synchronize": [],
+    "pkgpu_malloc": [ctypes.POINTER(_vp), _sz],
+    "pkgpu_free": [_vp],
+    "pkgpu_memcpy_h2d": [_vp, _vp, _sz, _vp],
+    "pkgpu_memcpy_d2h": [_vp, _vp, _sz, _vp],
+    "pkgpu_stream_synchronize": [_vp],
+    "pkgpu_ntt_nr": [_vp, _sz, _sz, _sz],
+    "pkgpu_ntt_nr_device": [_vp, _sz, _sz, _sz, _vp],
+    "pkgpu_intt_rn": [_vp, _sz],
+    "pkgpu_intt_rn_device": [_vp, _sz, _vp],
+    "pkgpu_rs_next_order": [_sz, ctypes.POINTER(_sz)],
+    "pkgpu_rs_generator": [_sz, _vp],
+    "pkgpu_rs_evaluation_points": [_sz, _sz, _vp, _sz, _vp],
+    "pkgpu_rs_interleaved_encode": [_vp, _sz, _sz, _vp, _sz, _sz, _vp],
+    "pkgpu_rs_interleaved_encode_device": [_vp, _sz, _sz, _vp, _sz, _sz, _i, _vp, _vp],
+    "pkgpu_skyscraper_compress": [_vp, _vp, _vp],
+    "pkgpu_skyscraper_compress_many": [_vp, _sz, _vp, _sz],
+    "pkgpu_skyscraper_hash_many": [_sz, _vp, _sz, _vp, _sz],
+    "pkgpu_skyscraper_hash_many_device": [_sz, _vp, _sz, _vp, _vp],
+    "pkgpu_skyscraper_merkle_device": [_vp, _sz, _vp, _vp],
+    "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
+    "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libprovekit_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ProvekitGpuError(
+                ERR_INTERNAL,
+                f"{LIB_PATH} is missing: build it with `python -m provekit_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.pkgpu_last_error.argtypes = []
+        L.pkgpu_last_error.restype = ctypes.c_char_p
+        L.pkgpu_abi_version.argtypes = []
+        L.pkgpu_abi_version.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().pkgpu_last_error().decode(errors="replace")
+        raise ProvekitGpuError(rc, msg)
+
+
+def _call(name: str, *args) -> None:
+    _check(getattr(lib(), name)(*args))
+
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    """pkgpu_init: select and set up the device for this thread."""
+    global _initialised
+    _call("pkgpu_init", device)
+    _initialised = True
+
+
+def _ensure_init() -> None:
+    if not _initialised:
+        init(int(os.environ.get("PKGPU_DEVICE", "0")))
+
+
+def _fr_array(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    if a.ndim == 1:
+        a = a.reshape(-1, 4)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("field arrays have shape (n, 4)")
+    return a
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ----------------------------------------------------------------- ntt crate
+def ntt_nr(values: np.ndarray, codeword_size: int, num_groups: int) -> None:
+    """ntt::ntt_nr (ntt/src/ntt.rs:105): in place, natural -> bit-reversed."""
+    _ensure_init()
+    if values.dtype != np.uint64 or not values.flags.c_contiguous:
+        raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
+    _call("pkgpu_ntt_nr", _ptr(values), values.size // 4, codeword_size, num_groups)
+
+
+def intt_rn(values: np.ndarray) -> None:
+    """ntt::intt_rn (ntt/src/ntt.rs:326): in place, bit-reversed -> natural."""
+    _ensure_init()
+    if values.dtype != np.uint64 or not values.flags.c_contiguous:
+        raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
+    _call("pkgpu_intt_rn", _ptr(values), values.size // 4)
+
+
+# ----------------------------------------------------------------- RSFr
+class RSFr:
+    """ReedSolomon<Fr> plugin RSFr (provekit/common/src/ntt.rs:12-105)."""
+
+    def next_order(self, size: int) -> Optional[int]:
+        out = _sz()
+        rc = lib().pkgpu_rs_next_order(size, ctypes.byref(out))
+        if rc == ERR_SIZE:
+            return None
+        _check(rc)
+        return out.value
+
+    def generator(self, codeword_length: int) -> np.ndarray:
+        out = np.zeros(4, dtype=np.uint64)
+        _call("pkgpu_rs_generator", codeword_length, _ptr(out))
+        return out
+
+    def evaluation_points(self, masked_message_length: int, codeword_length: int,
+                          indices: Sequence[int]) -> np.ndarray:
+        _ensure_init()
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        out = np.zeros((len(idx), 4), dtype=np.uint64)
+        _call("pkgpu_rs_evaluation_points", masked_message_length, codeword_length,
+              _ptr(idx), len(idx), _ptr(out))
+        return out
+
+    def interleaved_encode(self, messages: Sequence[np.ndarray], masks: np.ndarray,
+                           codeword_length: int) -> np.ndarray:
+        _ensure_init()
+        if len(messages) == 0:
+            return np.zeros((0, 4), dtype=np.uint64)
+        msgs = [_fr_array(m) for m in messages]
+        mlen = msgs[0].shape[0]
+        for m in msgs:
+            if m.shape[0] != mlen:
+                raise ProvekitGpuError(ERR_ARG, "assertion `left == right` failed: message lengths differ")
+        masks = _fr_array(masks) if masks is not None and len(masks) else np.zeros((0, 4), np.uint64)
+        ptrs = (ctypes.c_void_p * len(msgs))(*[_ptr(m) for m in msgs])
+        out = np.zeros((len(msgs) * codeword_length, 4), dtype=np.uint64)
+        _call("pkgpu_rs_interleaved_encode", ptrs, len(msgs), mlen, _ptr(masks),
+              masks.shape[0], codeword_length, _ptr(out))
+        return out
+
+
+# ----------------------------------------------------------------- Skyscraper
+class skyscraper:  # noqa: N801 - namespace mirroring the `skyscraper` crate
+    """skyscraper::simple (skyscraper/core/src/simple.rs)."""
+
+    @staticmethod
+    def compress(l, r) -> np.ndarray:
+        _ensure_init()
+        a = np.ascontiguousarray(l, dtype=np.uint64).reshape(4)
+        b = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+        out = np.zeros(4, dtype=np.uint64)
+        _call("pkgpu_skyscraper_compress", _ptr(a), _ptr(b), _ptr(out))
+        return out
+
+    @staticmethod
+    def compress_many(messages: bytes) -> bytes:
+        _ensure_init()
+        m = np.frombuffer(messages, dtype=np.uint8)
+        out = np.zeros(len(messages) // 2, dtype=np.uint8)
+        _call("pkgpu_skyscraper_compress_many", _ptr(m) if len(m) else None, len(m),
+              _ptr(out) if len(out) else None, len(out))
+        return out.tobytes()
+
+
+class SkyscraperHashEngine:
+    """whir HashEngine plugin (provekit/common/src/skyscraper/whir.rs:29-101)."""
+
+    def name(self) -> str:
+        return "skyscraper"
+
+    def supports_size(self, size: int) -> bool:
+        return size > 0 and size % 32 == 0
+
+    def preferred_batch_size(self) -> int:
+        return 1 << 16
+
+    def hash_many(self, size: int, input: bytes, count: int) -> bytes:
+        _ensure_init()
+        m = np.frombuffer(input, dtype=np.uint8)
+        out = np.zeros(32 * count, dtype=np.uint8)
+        _call("pkgpu_skyscraper_hash_many", size, _ptr(m) if len(m) else None, len(m),
+              _ptr(out) if len(out) else None, count)
+        return out.tobytes()
+
+
+# ----------------------------------------------------------------- commit
+def commit(coeffs: np.ndarray, blowup: int, coset: bool = True, leaf_elems: int = 16,
+           want_codeword: bool = True, want_nodes: bool = True):
+    """Commit one polynomial (pkgpu_commit).  Returns (root bytes, codeword or
+    None, nodes (2*leaves-1, 32) uint8 or None)."""
+    _ensure_init()
+    c = _fr_array(coeffs)
+    n = c.shape[0]
+    N = n * blowup
+    leaves = N // leaf_elems
+    cw = np.zeros((N, 4), dtype=np.uint64) if want_codeword else None
+    nodes = np.zeros((2 * leaves - 1, 32), dtype=np.uint8) if want_nodes else None
+    root = np.zeros(32, dtype=np.uint8)
+    _call("pkgpu_commit", _ptr(c), n, blowup, int(bool(coset)), leaf_elems,
+          _ptr(cw) if cw is not None else None, _ptr(nodes) if nodes is not None else None,
+          _ptr(root))
+    return root.tobytes(), cw, nodes
+
+
+# ----------------------------------------------------------------- device memory
+class DeviceBuffer:
+    """A device allocation (pkgpu_malloc) for the *_device entry points."""
+
+    def __init__(self, nbytes: int):
+        _ensure_init()
+        p = ctypes.c_void_p()
+        _call("pkgpu_malloc", ctypes.byref(p), nbytes)
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        _call("pkgpu_memcpy_h2d", self.ptr, _ptr(a), a.nbytes, stream)
+
+    def download(self, dtype=np.uint64, shape=None, nbytes=None, stream=None) -> np.ndarray:
+        nb = self.nbytes if nbytes is None else nbytes
+        out = np.empty(nb // np.dtype(dtype).itemsize, dtype=dtype)
+        _call("pkgpu_memcpy_d2h", _ptr(out), self.ptr, nb, stream)
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().pkgpu_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _call("pkgpu_device_synchronize")
+
+
+def commit_device(coeffs: DeviceBuffer, n: int, blowup: int, coset: bool, leaf_elems: int,
+                  codeword: DeviceBuffer, nodes: DeviceBuffer, stream=None) -> None:
+    _call("pkgpu_commit_device", coeffs.ptr, n, blowup, int(bool(coset)), leaf_elems,
+          codeword.ptr, nodes.ptr, stream)
+
+
+def ntt_nr_device(values: DeviceBuffer, length: int, codeword_size: int, num_groups: int,
+                  stream=None) -> None:
+    _call("pkgpu_ntt_nr_device", values.ptr, length, codeword_size, num_groups, stream)
+
+
+def hash_many_device(size: int, inp: DeviceBuffer, count: int, out: DeviceBuffer,
+                     stream=None) -> None:
+    _call("pkgpu_skyscraper_hash_many_device", size, inp.ptr, count, out.ptr, stream)
+
+
+def merkle_device(leaves: DeviceBuffer, n: int, nodes: DeviceBuffer, stream=None) -> None:
+    _call("pkgpu_skyscraper_merkle_device", leaves.ptr, n, nodes.ptr, stream)

@@ -1,0 +1,437 @@
+// C ABI (include/provekit_gpu.h) and the C++ host mirror (provekit.hpp).
+#include <string.h>
+
+#include <string>
+
+#include "../../include/provekit_gpu.h"
+#include "host_fr.hpp"
+#include "kernels.hpp"
+#include "provekit.hpp"
+
+using pk::Error;
+using pk::fr_t;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return PKGPU_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return PKGPU_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return PKGPU_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return PKGPU_ERR_INTERNAL;
+  }
+}
+
+hipStream_t stream_or_thread(void* s) {
+  return s ? static_cast<hipStream_t>(s) : pk::thread_stream();
+}
+
+// Host <-> device staging for the slice-taking entry points.
+struct Staged {
+  pk::Scratch buf;
+  Staged(size_t bytes, hipStream_t s) : buf(bytes, s) {}
+  template <class T>
+  T* get() const { return buf.as<T>(); }
+};
+
+void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
+  if (bytes) PK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+}
+void d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
+  if (bytes) PK_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+}
+void sync(hipStream_t s) { PK_HIP(hipStreamSynchronize(s)); }
+
+#define REQ(c, code, msg) PK_REQUIRE(c, code, msg)
+
+}  // namespace
+
+namespace provekit_gpu {
+
+// ntt/src/ntt.rs:105-108 with the asserts of interleaved_ntt_nr (:160-169)
+void ntt_nr(Fr* values, size_t len, size_t codeword_size, size_t num_groups) {
+  if (codeword_size <= 1) return;  // ntt.rs:160-162
+  REQ(num_groups >= 1 && len % num_groups == 0, pk::PK_ERR_SIZE,
+      "values.len() must be divisible by num_groups");
+  REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  REQ(pk::is_pow2(len) && len % codeword_size == 0 && pk::is_pow2(num_groups) &&
+          num_groups <= codeword_size,
+      pk::PK_ERR_SIZE, "ntt_nr: len must be a power-of-two multiple of codeword_size");
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged b(sizeof(fr_t) * len, s);
+  h2d(b.get<void>(), values, sizeof(fr_t) * len, s);
+  pk::ntt_nr_dev(d, b.get<fr_t>(), len, codeword_size, num_groups, s);
+  d2h(values, b.get<void>(), sizeof(fr_t) * len, s);
+  sync(s);
+}
+
+void intt_rn(Fr* values, size_t len) {
+  if (len <= 1) return;
+  REQ(pk::is_pow2(len), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged b(sizeof(fr_t) * len, s);
+  h2d(b.get<void>(), values, sizeof(fr_t) * len, s);
+  pk::intt_rn_dev(d, b.get<fr_t>(), len, s);
+  d2h(values, b.get<void>(), sizeof(fr_t) * len, s);
+  sync(s);
+}
+
+std::optional<size_t> RSFr::next_order(size_t size) const {
+  size_t o = 1;
+  while (o < size) o <<= 1;
+  if (o <= (size_t(1) << 28)) return o;
+  return std::nullopt;
+}
+
+Fr RSFr::generator(size_t codeword_length) const {
+  REQ(pk::is_pow2(codeword_length) && codeword_length <= (size_t(1) << 28), pk::PK_ERR_SIZE,
+      "get_root_of_unity: codeword length must be a power of two up to 2^28");
+  pk::hfr g = pk::hf::root_of_unity_log((unsigned)pk::log2_exact(codeword_length));
+  Fr r;
+  memcpy(r.data(), g.l, 32);
+  return r;
+}
+
+std::vector<Fr> RSFr::evaluation_points(size_t, size_t codeword_length,
+                                        const std::vector<size_t>& indices) const {
+  REQ(pk::is_pow2(codeword_length) && codeword_length <= (size_t(1) << 28), pk::PK_ERR_SIZE,
+      "get_root_of_unity: codeword length must be a power of two up to 2^28");
+  std::vector<Fr> out(indices.size());
+  if (indices.empty()) return out;
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged di(sizeof(uint64_t) * indices.size(), s);
+  Staged dout(sizeof(fr_t) * indices.size(), s);
+  std::vector<uint64_t> idx(indices.begin(), indices.end());
+  h2d(di.get<void>(), idx.data(), sizeof(uint64_t) * idx.size(), s);
+  pk::rs_points_dev(d, codeword_length, di.get<uint64_t>(), idx.size(), dout.get<fr_t>(), s);
+  d2h(out.data(), dout.get<void>(), sizeof(fr_t) * out.size(), s);
+  sync(s);
+  return out;
+}
+
+std::vector<Fr> RSFr::interleaved_encode(const std::vector<const Fr*>& messages,
+                                         size_t message_length, const std::vector<Fr>& masks,
+                                         size_t codeword_length) const {
+  if (messages.empty()) return {};
+  const size_t M = messages.size();
+  REQ(pk::is_pow2(codeword_length), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  REQ(M * message_length + masks.size() <= M * codeword_length, pk::PK_ERR_SIZE,
+      "interleaved_encode: messages and masks longer than the codeword");
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged dm(sizeof(fr_t) * M * message_length, s);
+  Staged dk(sizeof(fr_t) * masks.size(), s);
+  Staged dout(sizeof(fr_t) * M * codeword_length, s);
+  for (size_t r = 0; r < M; r++)
+    h2d(dm.get<fr_t>() + r * message_length, messages[r], sizeof(fr_t) * message_length, s);
+  h2d(dk.get<void>(), masks.data(), sizeof(fr_t) * masks.size(), s);
+  pk::rs_encode_dev(d, dm.get<fr_t>(), M, message_length, dk.get<fr_t>(), masks.size(),
+                    codeword_length, dout.get<fr_t>(), 0, s);
+  std::vector<Fr> out(M * codeword_length);
+  d2h(out.data(), dout.get<void>(), sizeof(fr_t) * out.size(), s);
+  sync(s);
+  return out;
+}
+
+namespace skyscraper {
+void compress_many(const uint8_t* messages, size_t messages_len, uint8_t* hashes,
+                   size_t hashes_len) {
+  REQ(messages_len % 64 == 0, pk::PK_ERR_SIZE, "Message length not a multiple of 64");
+  REQ(hashes_len % 32 == 0, pk::PK_ERR_SIZE, "Hashes length not a multiple of 32");
+  REQ(messages_len / 64 == hashes_len / 32, pk::PK_ERR_SIZE,
+      "Messages and hashes length mismatch");
+  const size_t n = hashes_len / 32;
+  if (!n) return;
+  hipStream_t s = pk::thread_stream();
+  Staged dm(messages_len, s), dh(hashes_len, s);
+  h2d(dm.get<void>(), messages, messages_len, s);
+  pk::sky_compress_many_dev(dm.get<fr_t>(), dh.get<fr_t>(), n, s);
+  d2h(hashes, dh.get<void>(), hashes_len, s);
+  sync(s);
+}
+Fr compress(const Fr& l, const Fr& r) {
+  uint8_t m[64];
+  memcpy(m, l.data(), 32);
+  memcpy(m + 32, r.data(), 32);
+  Fr o;
+  compress_many(m, 64, reinterpret_cast<uint8_t*>(o.data()), 32);
+  return o;
+}
+}  // namespace skyscraper
+
+void SkyscraperHashEngine::hash_many(size_t size, const uint8_t* input, size_t input_len,
+                                     Hash* output, size_t count) const {
+  REQ(supports_size(size), pk::PK_ERR_SIZE,
+      "skyscraper: unsupported message size " + std::to_string(size) +
+          " (must be a positive multiple of 32)");
+  REQ(input_len == size * count, pk::PK_ERR_SIZE,
+      "skyscraper: input length " + std::to_string(input_len) + " != size " +
+          std::to_string(size) + " * count " + std::to_string(count));
+  if (!count) return;
+  if (size == 32) {
+    memcpy(output, input, input_len);
+    return;
+  }
+  hipStream_t s = pk::thread_stream();
+  Staged di(input_len, s), dout(32 * count, s);
+  h2d(di.get<void>(), input, input_len, s);
+  pk::sky_hash_many_dev(size / 32, di.get<fr_t>(), count, dout.get<fr_t>(), s);
+  d2h(output, dout.get<void>(), 32 * count, s);
+  sync(s);
+}
+
+Hash commit(const Fr* coeffs, size_t n, size_t blowup, bool coset, size_t leaf_elems,
+            Fr* codeword_out, Hash* nodes_out) {
+  REQ(pk::is_pow2(n) && pk::is_pow2(blowup), pk::PK_ERR_NOT_POW2,
+      "commit: length and blowup must be powers of two");
+  REQ(leaf_elems >= 1 && pk::is_pow2(leaf_elems) && (n * blowup) % leaf_elems == 0,
+      pk::PK_ERR_SIZE, "commit: leaf size must be a power of two dividing the codeword");
+  const size_t N = n * blowup, leaves = N / leaf_elems, nodes = 2 * leaves - 1;
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged dc(sizeof(fr_t) * n, s), dw(sizeof(fr_t) * N, s), dn(sizeof(fr_t) * nodes, s);
+  h2d(dc.get<void>(), coeffs, sizeof(fr_t) * n, s);
+  pk::commit_dev(d, dc.get<fr_t>(), n, blowup, coset ? 1 : 0, leaf_elems, dw.get<fr_t>(),
+                 dn.get<fr_t>(), s);
+  Hash root;
+  d2h(root.data(), dn.get<fr_t>() + (nodes - 1), 32, s);
+  if (codeword_out) d2h(codeword_out, dw.get<void>(), sizeof(fr_t) * N, s);
+  if (nodes_out) d2h(nodes_out, dn.get<void>(), 32 * nodes, s);
+  sync(s);
+  return root;
+}
+
+}  // namespace provekit_gpu
+
+// ================================================================== C ABI
+using namespace provekit_gpu;
+
+extern "C" {
+
+int pkgpu_abi_version(void) { return PKGPU_ABI_VERSION; }
+
+const char* pkgpu_last_error(void) { return g_last_error.c_str(); }
+
+int pkgpu_init(int device) {
+  return guarded([&] {
+    int n = 0;
+    PK_HIP(hipGetDeviceCount(&n));
+    REQ(device >= 0 && device < n, pk::PK_ERR_ARG, "pkgpu_init: no such device");
+    PK_HIP(hipSetDevice(device));
+    pk::device_get(device);
+    (void)pk::thread_stream();
+  });
+}
+
+int pkgpu_device_synchronize(void) {
+  return guarded([&] { PK_HIP(hipDeviceSynchronize()); });
+}
+
+int pkgpu_malloc(void** ptr, size_t bytes) {
+  return guarded([&] {
+    REQ(ptr, pk::PK_ERR_ARG, "pkgpu_malloc: null out-pointer");
+    PK_HIP(hipMalloc(ptr, bytes ? bytes : 16));
+  });
+}
+int pkgpu_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) PK_HIP(hipFree(ptr));
+  });
+}
+int pkgpu_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return guarded([&] {
+    REQ(bytes == 0 || (dst && src), pk::PK_ERR_ARG, "memcpy: null pointer");
+    hipStream_t s = stream_or_thread(stream);
+    h2d(dst, src, bytes, s);
+    sync(s);
+  });
+}
+int pkgpu_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  return guarded([&] {
+    REQ(bytes == 0 || (dst && src), pk::PK_ERR_ARG, "memcpy: null pointer");
+    hipStream_t s = stream_or_thread(stream);
+    d2h(dst, src, bytes, s);
+    sync(s);
+  });
+}
+int pkgpu_stream_synchronize(void* stream) {
+  return guarded([&] { sync(stream_or_thread(stream)); });
+}
+
+int pkgpu_ntt_nr(uint64_t* values, size_t len, size_t codeword_size, size_t num_groups) {
+  return guarded([&] {
+    REQ(len == 0 || values, pk::PK_ERR_ARG, "ntt_nr: null values");
+    ntt_nr(reinterpret_cast<Fr*>(values), len, codeword_size, num_groups);
+  });
+}
+int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size_t num_groups,
+                        void* stream) {
+  return guarded([&] {
+    if (codeword_size <= 1) return;
+    REQ(values, pk::PK_ERR_ARG, "ntt_nr: null values");
+    REQ(num_groups >= 1 && len % num_groups == 0, pk::PK_ERR_SIZE,
+        "values.len() must be divisible by num_groups");
+    REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+    REQ(pk::is_pow2(len) && len % codeword_size == 0 && pk::is_pow2(num_groups) &&
+            num_groups <= codeword_size,
+        pk::PK_ERR_SIZE, "ntt_nr: len must be a power-of-two multiple of codeword_size");
+    pk::ntt_nr_dev(pk::device_current(), reinterpret_cast<fr_t*>(values), len, codeword_size,
+                   num_groups, stream_or_thread(stream));
+  });
+}
+int pkgpu_intt_rn(uint64_t* values, size_t len) {
+  return guarded([&] {
+    REQ(len == 0 || values, pk::PK_ERR_ARG, "intt_rn: null values");
+    intt_rn(reinterpret_cast<Fr*>(values), len);
+  });
+}
+int pkgpu_intt_rn_device(uint64_t* values, size_t len, void* stream) {
+  return guarded([&] {
+    if (len <= 1) return;
+    REQ(values, pk::PK_ERR_ARG, "intt_rn: null values");
+    pk::intt_rn_dev(pk::device_current(), reinterpret_cast<fr_t*>(values), len,
+                    stream_or_thread(stream));
+  });
+}
+
+int pkgpu_rs_next_order(size_t size, size_t* order) {
+  return guarded([&] {
+    REQ(order, pk::PK_ERR_ARG, "next_order: null out-pointer");
+    auto o = RSFr().next_order(size);
+    REQ(o.has_value(), pk::PK_ERR_SIZE, "next_order: order above 2^28");
+    *order = *o;
+  });
+}
+int pkgpu_rs_generator(size_t codeword_length, uint64_t out[4]) {
+  return guarded([&] {
+    REQ(out, pk::PK_ERR_ARG, "generator: null out");
+    Fr g = RSFr().generator(codeword_length);
+    memcpy(out, g.data(), 32);
+  });
+}
+int pkgpu_rs_evaluation_points(size_t masked_message_length, size_t codeword_length,
+                               const size_t* indices, size_t n, uint64_t* out) {
+  return guarded([&] {
+    REQ(n == 0 || (indices && out), pk::PK_ERR_ARG, "evaluation_points: null pointer");
+    std::vector<size_t> idx(indices, indices + n);
+    auto pts = RSFr().evaluation_points(masked_message_length, codeword_length, idx);
+    if (n) memcpy(out, pts.data(), 32 * n);
+  });
+}
+int pkgpu_rs_interleaved_encode(const uint64_t* const* messages, size_t num_messages,
+                                size_t message_length, const uint64_t* masks, size_t masks_len,
+                                size_t codeword_length, uint64_t* out) {
+  return guarded([&] {
+    if (num_messages == 0) return;
+    REQ(messages && out && (masks_len == 0 || masks), pk::PK_ERR_ARG,
+        "interleaved_encode: null pointer");
+    std::vector<const Fr*> msgs(num_messages);
+    for (size_t r = 0; r < num_messages; r++) {
+      REQ(message_length == 0 || messages[r], pk::PK_ERR_ARG, "interleaved_encode: null message");
+      msgs[r] = reinterpret_cast<const Fr*>(messages[r]);
+    }
+    std::vector<Fr> mk(masks_len);
+    if (masks_len) memcpy(mk.data(), masks, 32 * masks_len);
+    auto res = RSFr().interleaved_encode(msgs, message_length, mk, codeword_length);
+    memcpy(out, res.data(), 32 * res.size());
+  });
+}
+int pkgpu_rs_interleaved_encode_device(const uint64_t* messages, size_t num_messages,
+                                       size_t message_length, const uint64_t* masks,
+                                       size_t masks_len, size_t codeword_length, int coset,
+                                       uint64_t* out, void* stream) {
+  return guarded([&] {
+    if (num_messages == 0) return;
+    REQ(messages && out && (masks_len == 0 || masks), pk::PK_ERR_ARG,
+        "interleaved_encode: null pointer");
+    REQ(num_messages * message_length + masks_len <= num_messages * codeword_length,
+        pk::PK_ERR_SIZE, "interleaved_encode: messages and masks longer than the codeword");
+    pk::rs_encode_dev(pk::device_current(), reinterpret_cast<const fr_t*>(messages), num_messages,
+                      message_length, reinterpret_cast<const fr_t*>(masks), masks_len,
+                      codeword_length, reinterpret_cast<fr_t*>(out), coset,
+                      stream_or_thread(stream));
+  });
+}
+
+int pkgpu_skyscraper_compress(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]) {
+  return guarded([&] {
+    REQ(l && r && out, pk::PK_ERR_ARG, "compress: null pointer");
+    Fr a, b;
+    memcpy(a.data(), l, 32);
+    memcpy(b.data(), r, 32);
+    Fr o = skyscraper::compress(a, b);
+    memcpy(out, o.data(), 32);
+  });
+}
+int pkgpu_skyscraper_compress_many(const uint8_t* messages, size_t messages_len,
+                                   uint8_t* hashes, size_t hashes_len) {
+  return guarded([&] {
+    REQ(hashes_len == 0 || (messages && hashes), pk::PK_ERR_ARG, "compress_many: null pointer");
+    skyscraper::compress_many(messages, messages_len, hashes, hashes_len);
+  });
+}
+int pkgpu_skyscraper_hash_many(size_t size, const uint8_t* input, size_t input_len,
+                               uint8_t* output, size_t count) {
+  return guarded([&] {
+    REQ(count == 0 || (input && output), pk::PK_ERR_ARG, "hash_many: null pointer");
+    SkyscraperHashEngine().hash_many(size, input, input_len, reinterpret_cast<Hash*>(output),
+                                     count);
+  });
+}
+int pkgpu_skyscraper_hash_many_device(size_t size, const void* input, size_t count, void* output,
+                                      void* stream) {
+  return guarded([&] {
+    REQ(size > 0 && size % 32 == 0, pk::PK_ERR_SIZE,
+        "skyscraper: unsupported message size " + std::to_string(size) +
+            " (must be a positive multiple of 32)");
+    if (!count) return;
+    REQ(input && output, pk::PK_ERR_ARG, "hash_many: null pointer");
+    pk::sky_hash_many_dev(size / 32, static_cast<const fr_t*>(input), count,
+                          static_cast<fr_t*>(output), stream_or_thread(stream));
+  });
+}
+int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, void* stream) {
+  return guarded([&] {
+    REQ(leaves && nodes, pk::PK_ERR_ARG, "merkle: null pointer");
+    pk::sky_merkle_dev(static_cast<const fr_t*>(leaves), n, static_cast<fr_t*>(nodes),
+                       stream_or_thread(stream));
+  });
+}
+
+int pkgpu_commit(const uint64_t* coeffs, size_t n, size_t blowup, int coset, size_t leaf_elems,
+                 uint64_t* codeword, uint8_t* nodes, uint8_t root[32]) {
+  return guarded([&] {
+    REQ(coeffs && root, pk::PK_ERR_ARG, "commit: null pointer");
+    Hash h = commit(reinterpret_cast<const Fr*>(coeffs), n, blowup, coset != 0, leaf_elems,
+                    reinterpret_cast<Fr*>(codeword), reinterpret_cast<Hash*>(nodes));
+    memcpy(root, h.data(), 32);
+  });
+}
+int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int coset,
+                        size_t leaf_elems, uint64_t* codeword, void* nodes, void* stream) {
+  return guarded([&] {
+    REQ(coeffs && codeword && nodes, pk::PK_ERR_ARG, "commit: null pointer");
+    pk::commit_dev(pk::device_current(), reinterpret_cast<const fr_t*>(coeffs), n, blowup, coset,
+                   leaf_elems, reinterpret_cast<fr_t*>(codeword), static_cast<fr_t*>(nodes),
+                   stream_or_thread(stream));
+  });
+}
+
+}  // extern "C"

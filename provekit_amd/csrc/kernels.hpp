@@ -1,0 +1,44 @@
+// Launchers for the gfx950 kernels.  All pointers are device pointers; every
+// launcher enqueues on `s` and returns without synchronising.
+#pragma once
+#include "runtime.hpp"
+
+namespace pk {
+
+// ---- NTT (ntt.hip) ----
+// In-place DIT NTT, natural -> bit-reversed order, over `len` elements of
+// `len / codeword` interleaved polynomials, starting at stage log2(num_groups)
+// (ntt::ntt_nr).  If `src` is given, the first pass reads element g from
+// src[g % chunk] instead of data[g] (replication of the first coset chunk).
+void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
+                hipStream_t s, const fr_t* src = nullptr, uint64_t chunk = 0);
+// ntt::intt_rn on device (uses one scratch buffer of len elements).
+void intt_rn_dev(Device& d, fr_t* data, uint64_t len, hipStream_t s);
+// Bit-reversal permutation of n elements, out-of-place.
+void bitrev_permute_dev(const fr_t* in, fr_t* out, uint64_t n, hipStream_t s);
+// RSFr::interleaved_encode into `out` (num_messages * codeword elements).
+// msgs: num_messages rows of msg_len (row-major).  coset != 0 evaluates on the
+// coset 5*H instead of H (coefficient of index j scaled by 5^j).
+void rs_encode_dev(Device& d, const fr_t* msgs, uint64_t num_messages, uint64_t msg_len,
+                   const fr_t* masks, uint64_t masks_len, uint64_t codeword, fr_t* out,
+                   int coset, hipStream_t s);
+// Evaluation points w^{bitrev(i)} for the given indices (RSFr::evaluation_points).
+void rs_points_dev(Device& d, uint64_t codeword, const uint64_t* idx, uint64_t n, fr_t* out,
+                   hipStream_t s);
+
+// ---- Skyscraper (skyscraper.hip) ----
+void sky_compress_many_dev(const fr_t* pairs, fr_t* out, uint64_t n, hipStream_t s);
+// SkyscraperHashEngine::hash_many for messages of `chunks` 32-byte words.
+void sky_hash_many_dev(uint64_t chunks, const fr_t* in, uint64_t count, fr_t* out, hipStream_t s);
+// Binary tree over n (power of two) leaf digests: nodes[0..n) = leaves (may
+// alias `leaves`), levels follow, root at nodes[2n-2].
+void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s);
+
+// ---- commit pipeline (commit.hip) ----
+// One polynomial of n coefficients: RS encode at `blowup` (optionally on the
+// coset 5*H), leaves of `leaf_elems` consecutive codeword elements hashed by
+// left fold, binary Skyscraper tree.  codeword: n*blowup; nodes: 2*leaves-1.
+void commit_dev(Device& d, const fr_t* coeffs, uint64_t n, uint64_t blowup, int coset,
+                uint64_t leaf_elems, fr_t* codeword, fr_t* nodes, hipStream_t s);
+
+}  // namespace pk

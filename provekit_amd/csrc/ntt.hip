@@ -1,0 +1,420 @@
+// Reed-Solomon encoding by NTT over BN254 Fr on gfx950.
+//
+// Algorithm (same transform as ntt/src/ntt.rs:150-248): decimation in time,
+// natural input order, bit-reversed output.  Stage s (s = 0 .. log2(codeword)-1)
+// splits the array into 2^s groups; group k pairs (i, i + half) with twiddle
+// rr[k], rr[j] = w^{bitrev(j)}.  Exact modular arithmetic makes the result
+// independent of how the stages are scheduled.
+//
+// Schedule on the GPU: the stages are cut into passes of up to 11 stages.  A
+// pass covering stages [s0, s0 + r) touches index bits [q, q + r) with
+// q = L - s0 - r; a workgroup stages a tile of 2^r "window" positions times
+// 2^jb consecutive low indices (coalesced runs of 2^jb * 32 bytes) in LDS,
+// runs its r stages there with one barrier per stage, and writes the tile
+// back.  Elements stay in the lazy range [0, 2p) between stages and are
+// made canonical in the last pass.
+#include "kernels.hpp"
+#include "host_fr.hpp"
+
+namespace pk {
+
+namespace {
+
+__device__ __forceinline__ uint64_t bitrev(uint64_t v, int bits) {
+  return bits == 0 ? 0 : (__builtin_bitreverse64(v) >> (64 - bits));
+}
+
+// out[i] = base^i, i < n, by square-and-multiply per element (set-up only).
+__global__ void k_pow_table(fr_t* out, fr_t base, uint64_t n, fr_t one) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t r = one, b = base;
+  uint64_t e = i;
+  while (e) {
+    if (e & 1) r = fr_mul(r, b);
+    b = fr_mul(b, b);
+    e >>= 1;
+  }
+  fr_canon(r);
+  out[i] = r;
+}
+
+// out[j] = lo[e & mask] * hi[e >> lo_bits] with e = bitrev_{bits}(j) if
+// bitrev_bits > 0, else e = j.  lo/hi hold base^i and base^(i << lo_bits).
+__global__ void k_two_level(fr_t* out, const fr_t* lo, const fr_t* hi, int lo_bits, uint64_t n,
+                            int bitrev_bits) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint64_t e = bitrev_bits > 0 ? bitrev(j, bitrev_bits) : j;
+  fr_t v = fr_mul(lo[e & ((1ull << lo_bits) - 1)], hi[e >> lo_bits]);
+  fr_canon(v);
+  out[j] = v;
+}
+
+// Build a table of 2^log entries of base^{e(j)} (e = bitrev if requested).
+void build_power_table(fr_t* out, const hfr& base_h, int log, bool bit_reversed, hipStream_t s) {
+  uint64_t n = 1ull << log;
+  int lo_bits = (log + 1) / 2;
+  int hi_bits = log - lo_bits;
+  fr_t *lo, *hi;
+  PK_HIP(hipMallocAsync((void**)&lo, sizeof(fr_t) << lo_bits, s));
+  PK_HIP(hipMallocAsync((void**)&hi, sizeof(fr_t) << hi_bits, s));
+  fr_t base, one, step;
+  memcpy(&base, &base_h, 32);
+  memcpy(&one, &hf::ONE, 32);
+  hfr step_h = base_h;
+  for (int i = 0; i < lo_bits; i++) step_h = hf::mul(step_h, step_h);
+  memcpy(&step, &step_h, 32);
+  uint64_t nlo = 1ull << lo_bits, nhi = 1ull << hi_bits;
+  k_pow_table<<<(unsigned)((nlo + 255) / 256), 256, 0, s>>>(lo, base, nlo, one);
+  k_pow_table<<<(unsigned)((nhi + 255) / 256), 256, 0, s>>>(hi, step, nhi, one);
+  k_two_level<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(out, lo, hi, lo_bits, n,
+                                                          bit_reversed ? log : 0);
+  PK_HIP(hipGetLastError());
+  PK_HIP(hipFreeAsync(lo, s));
+  PK_HIP(hipFreeAsync(hi, s));
+}
+
+struct PassArgs {
+  fr_t* data;
+  const fr_t* src;
+  const fr_t* rr;
+  uint64_t chunk;
+  int L, s0, r, jb, canon;
+};
+
+template <int TL>
+struct PassShape {
+  static constexpr int T = (TL >= 3) ? ((1 << (TL - 3)) > 256 ? 256 : (1 << (TL - 3))) : 1;
+  static constexpr int BF = (TL >= 1) ? (1 << (TL - 1)) : 0;  // butterflies per stage
+  static constexpr int BPT = BF / T;
+};
+
+template <int TL>
+__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
+  constexpr int T = PassShape<TL>::T;
+  constexpr int BPT = PassShape<TL>::BPT;
+  constexpr int N = 1 << TL;
+  __shared__ fr_t lds[N];
+  const int tid = threadIdx.x;
+  const int q = a.L - a.s0 - a.r;
+  const int jb = a.jb;
+  const uint64_t tiles_per_h = (1ull << q) >> jb;
+  const uint64_t tile = blockIdx.x;
+  const uint64_t h = tile / tiles_per_h;
+  const uint64_t j0 = (tile % tiles_per_h) << jb;
+  const uint64_t base = (h << (a.L - a.s0)) | j0;
+  const uint32_t jmask = (1u << jb) - 1;
+
+  for (int e = tid; e < N; e += T) {
+    const uint64_t g = base + ((uint64_t)(e >> jb) << q) + (e & jmask);
+    fr_t v = a.src ? fr_load(a.src + (g % a.chunk)) : fr_load(a.data + g);
+    lds[e] = v;
+  }
+  __syncthreads();
+
+  for (int t = 0; t < a.r; t++) {
+    const int pos = a.r - 1 - t;
+    const int span = 1 << (pos + jb);
+    fr_t tw[BPT], x[BPT], y[BPT];
+    int e0[BPT];
+#pragma unroll
+    for (int k = 0; k < BPT; k++) {
+      const int b = tid + k * T;
+      const int jl = b & jmask;
+      const int wb = b >> jb;
+      const int w0 = ((wb >> pos) << (pos + 1)) | (wb & ((1 << pos) - 1));
+      e0[k] = (w0 << jb) | jl;
+      const uint64_t kt = (h << t) | ((uint64_t)w0 >> (a.r - t));
+      tw[k] = fr_load(a.rr + kt);
+      x[k] = lds[e0[k]];
+      y[k] = lds[e0[k] + span];
+    }
+#pragma unroll
+    for (int k = 0; k < BPT; k++) {
+      const fr_t m = fr_mul(tw[k], y[k]);
+      lds[e0[k]] = fr_add(x[k], m);
+      lds[e0[k] + span] = fr_sub(x[k], m);
+    }
+    __syncthreads();
+  }
+
+  for (int e = tid; e < N; e += T) {
+    const uint64_t g = base + ((uint64_t)(e >> jb) << q) + (e & jmask);
+    fr_t v = lds[e];
+    if (a.canon) fr_canon(v);
+    fr_store(a.data + g, v);
+  }
+}
+
+template <int TL>
+void launch_pass(const PassArgs& a, uint64_t len, hipStream_t s) {
+  const uint64_t tiles = len >> TL;
+  k_ntt_pass<TL><<<(unsigned)tiles, PassShape<TL>::T, 0, s>>>(a);
+}
+
+void launch_pass_tl(int tl, const PassArgs& a, uint64_t len, hipStream_t s) {
+  switch (tl) {
+    case 1: launch_pass<1>(a, len, s); break;
+    case 2: launch_pass<2>(a, len, s); break;
+    case 3: launch_pass<3>(a, len, s); break;
+    case 4: launch_pass<4>(a, len, s); break;
+    case 5: launch_pass<5>(a, len, s); break;
+    case 6: launch_pass<6>(a, len, s); break;
+    case 7: launch_pass<7>(a, len, s); break;
+    case 8: launch_pass<8>(a, len, s); break;
+    case 9: launch_pass<9>(a, len, s); break;
+    case 10: launch_pass<10>(a, len, s); break;
+    case 11: launch_pass<11>(a, len, s); break;
+    default: throw Error(PK_ERR_INTERNAL, "ntt: bad tile size");
+  }
+  PK_HIP(hipGetLastError());
+}
+
+// Copy with replication: data[g] = src[g % chunk].
+__global__ void k_replicate(fr_t* data, const fr_t* src, uint64_t chunk, uint64_t len) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < len) fr_store(data + g, fr_load(src + g % chunk));
+}
+
+}  // namespace
+
+struct NttPass {
+  int s0, r, jb;
+};
+
+// Pass plan for stages [s_begin, s_end) of a length-2^L array.  The last pass
+// takes as many stages as fit a full tile; earlier passes split the rest
+// evenly, each keeping at least 2^kMinRun consecutive elements per run.
+static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end) {
+  constexpr int kTileLog = 11, kMinRun = 2;
+  std::vector<NttPass> out;
+  int rem = s_end - s_begin;
+  if (rem <= 0) return out;
+  int r_fin = rem < kTileLog ? rem : kTileLog;
+  int rest = rem - r_fin;
+  int nfirst = (rest + (kTileLog - kMinRun) - 1) / (kTileLog - kMinRun);
+  int s0 = s_begin;
+  for (int i = 0; i < nfirst; i++) {
+    int r = rest / nfirst + (i < rest % nfirst ? 1 : 0);
+    int q = L - s0 - r;
+    int jb = q < (kTileLog - r) ? q : (kTileLog - r);
+    out.push_back({s0, r, jb});
+    s0 += r;
+  }
+  int q = L - s_end;
+  int jb = q < (kTileLog - r_fin) ? q : (kTileLog - r_fin);
+  out.push_back({s0, r_fin, jb});
+  return out;
+}
+
+void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
+                hipStream_t s, const fr_t* src, uint64_t chunk) {
+  PK_REQUIRE(is_pow2(len), PK_ERR_NOT_POW2, "ntt: length must be a power of two");
+  const int L = log2_exact(len);
+  const int s_begin = log2_exact(num_groups);
+  const int s_end = codeword > 1 ? log2_exact(codeword) : 0;
+  std::vector<NttPass> plan = plan_passes(L, s_begin, s_end);
+  if (plan.empty()) {
+    if (src) {
+      k_replicate<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(data, src, chunk, len);
+      PK_HIP(hipGetLastError());
+    }
+    return;
+  }
+  const fr_t* rr = roots_table(d, s_end - 1);
+  for (size_t i = 0; i < plan.size(); i++) {
+    PassArgs a;
+    a.data = data;
+    a.src = (i == 0) ? src : nullptr;
+    a.chunk = chunk;
+    a.rr = rr;
+    a.L = L;
+    a.s0 = plan[i].s0;
+    a.r = plan[i].r;
+    a.jb = plan[i].jb;
+    a.canon = (i + 1 == plan.size());
+    launch_pass_tl(plan[i].r + plan[i].jb, a, len, s);
+  }
+}
+
+namespace {
+// y[i] = x[bitrev((n - i) mod n)]: the reverse_order then values[1..].reverse()
+// of intt_rn/intt_nr (ntt/src/ntt.rs:326-341) as one gather.
+__global__ void k_intt_in(const fr_t* x, fr_t* y, uint64_t n, int bits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = (n - i) & (n - 1);
+  fr_store(y + i, fr_load(x + bitrev(k, bits)));
+}
+// out[i] = f * w[bitrev(i)]: the 1/n scaling then reverse_order.
+__global__ void k_intt_out(const fr_t* w, fr_t* out, uint64_t n, int bits, fr_t f) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_t v = fr_mul(f, fr_load(w + bitrev(i, bits)));
+  fr_canon(v);
+  fr_store(out + i, v);
+}
+__global__ void k_bitrev(const fr_t* x, fr_t* y, uint64_t n, int bits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fr_store(y + i, fr_load(x + bitrev(i, bits)));
+}
+}  // namespace
+
+void bitrev_permute_dev(const fr_t* in, fr_t* out, uint64_t n, hipStream_t s) {
+  PK_REQUIRE(is_pow2(n), PK_ERR_NOT_POW2, "reverse_order: size must be a power of two");
+  k_bitrev<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, out, n, log2_exact(n));
+  PK_HIP(hipGetLastError());
+}
+
+void intt_rn_dev(Device& d, fr_t* data, uint64_t n, hipStream_t s) {
+  if (n <= 1) return;  // 1/1 scaling of a single element is the identity
+  PK_REQUIRE(is_pow2(n), PK_ERR_NOT_POW2, "intt_rn: length must be a power of two");
+  const int bits = log2_exact(n);
+  Scratch tmp(sizeof(fr_t) * n, s);
+  fr_t* w = tmp.as<fr_t>();
+  const unsigned g = (unsigned)((n + 255) / 256);
+  k_intt_in<<<g, 256, 0, s>>>(data, w, n, bits);
+  PK_HIP(hipGetLastError());
+  ntt_nr_dev(d, w, n, n, 1, s);
+  hfr inv_n = hf::inv(hf::from_u64(n));
+  fr_t f;
+  memcpy(&f, &inv_n, 32);
+  k_intt_out<<<g, 256, 0, s>>>(w, data, n, bits, f);
+  PK_HIP(hipGetLastError());
+}
+
+namespace {
+// The first coset chunk of RSFr::interleaved_encode
+// (provekit/common/src/ntt.rs:70-84): message coefficients interleaved
+// column-major, masks appended, zeros up to the chunk length.  With a coset
+// table, coefficient index i / M is scaled by 5^(i / M).
+__global__ void k_rs_chunk(fr_t* chunk, const fr_t* msgs, uint64_t M, uint64_t msg_len,
+                           const fr_t* masks, uint64_t masks_len, uint64_t chunk_len,
+                           const fr_t* coset) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= chunk_len) return;
+  const uint64_t nm = msg_len * M;
+  fr_t v;
+  if (i < nm) {
+    v = fr_load(msgs + (i % M) * msg_len + i / M);
+  } else if (i < nm + masks_len) {
+    v = fr_load(masks + (i - nm));
+  } else {
+    for (int k = 0; k < 8; k++) v.l[k] = 0;
+  }
+  if (coset && i < nm + masks_len) {
+    v = fr_mul(v, fr_load(coset + i / M));
+    fr_canon(v);
+  }
+  fr_store(chunk + i, v);
+}
+__global__ void k_rs_points(const fr_t* rr, const uint64_t* idx, uint64_t n, fr_t* out,
+                            int bits, fr_t one) {
+  // w^{bitrev_bits(i)} = rr-table entry... computed by square-and-multiply
+  // from w = rr[1]^... is not prefix-friendly; use the table of the full
+  // order: rr[j] = w^{bitrev_{bits-1}(j)} covers exponents below n/2, and
+  // w^{k + n/2} = -w^k.
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t N = 1ull << bits;
+  uint64_t i = idx[t];
+  uint64_t k = bits == 0 ? i : bitrev(i & (N - 1), bits);
+  // exponent k mod N (points repeat with period N)
+  k &= N - 1;
+  fr_t v;
+  if (bits == 0) {
+    v = one;
+  } else {
+    const uint64_t halfN = N >> 1;
+    const uint64_t kk = k & (halfN - 1);
+    v = rr[bits >= 2 ? bitrev(kk, bits - 1) : 0];
+    if (k >= halfN) {
+      fr_t z;
+      for (int q = 0; q < 8; q++) z.l[q] = 0;
+      v = fr_sub_canon(z, v);
+    }
+  }
+  out[t] = v;
+}
+}  // namespace
+
+void rs_points_dev(Device& d, uint64_t codeword, const uint64_t* idx, uint64_t n, fr_t* out,
+                   hipStream_t s) {
+  PK_REQUIRE(codeword >= 1 && is_pow2(codeword), PK_ERR_NOT_POW2,
+             "evaluation_points: codeword length must be a power of two");
+  if (n == 0) return;
+  const int bits = log2_exact(codeword);
+  fr_t one;
+  memcpy(&one, &hf::ONE, 32);
+  const fr_t* rr = roots_table(d, bits >= 1 ? bits - 1 : 0);
+  k_rs_points<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(rr, idx, n, out, bits, one);
+  PK_HIP(hipGetLastError());
+}
+
+void rs_encode_dev(Device& d, const fr_t* msgs, uint64_t M, uint64_t msg_len, const fr_t* masks,
+                   uint64_t masks_len, uint64_t codeword, fr_t* out, int coset, hipStream_t s) {
+  if (M == 0) return;
+  PK_REQUIRE(is_pow2(codeword), PK_ERR_NOT_POW2, "interleaved_encode: codeword length must be a power of two");
+  const uint64_t mask_len = masks_len / M;
+  uint64_t coset_size = 1;
+  while (coset_size < msg_len + mask_len) coset_size <<= 1;
+  PK_REQUIRE(coset_size <= (1ull << 28), PK_ERR_SIZE, "interleaved_encode: order above 2^28");
+  while (codeword % coset_size) coset_size <<= 1;
+  PK_REQUIRE(coset_size <= codeword, PK_ERR_SIZE,
+             "interleaved_encode: message longer than the codeword");
+  const uint64_t num_cosets = codeword / coset_size;
+  const uint64_t chunk_len = coset_size * M;
+  const uint64_t total = codeword * M;
+  const fr_t* src = msgs;
+  Scratch chunk;
+  bool direct = (M == 1 && masks_len == 0 && !coset && msg_len == coset_size);
+  if (!direct) {
+    chunk = Scratch(sizeof(fr_t) * chunk_len, s);
+    const fr_t* cpow = nullptr;
+    if (coset) {
+      uint64_t need = msg_len + mask_len;
+      int lg = 0;
+      while ((1ull << lg) < need) lg++;
+      cpow = coset_table(d, lg);
+    }
+    k_rs_chunk<<<(unsigned)((chunk_len + 255) / 256), 256, 0, s>>>(
+        chunk.as<fr_t>(), msgs, M, msg_len, masks, masks_len, chunk_len, cpow);
+    PK_HIP(hipGetLastError());
+    src = chunk.as<fr_t>();
+  }
+  ntt_nr_dev(d, out, total, codeword, num_cosets, s, src, chunk_len);
+}
+
+// ---- tables (declared in runtime.hpp) ----
+static const fr_t* grow(Device& d, GrowTable& t, int log, int min_log, int max_log,
+                        const hfr& base, bool bit_reversed, const char* what) {
+  std::lock_guard<std::mutex> lk(d.tables_mtx);
+  if (t.log < log) {
+    int lg = log < min_log ? min_log : log;
+    PK_REQUIRE(lg <= max_log, PK_ERR_SIZE, what);
+    fr_t* p;
+    PK_HIP(hipMalloc((void**)&p, sizeof(fr_t) << lg));
+    hipStream_t bs;
+    PK_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+    build_power_table(p, bit_reversed ? hf::root_of_unity_log(lg + 1) : base, lg, bit_reversed, bs);
+    PK_HIP(hipStreamSynchronize(bs));
+    PK_HIP(hipStreamDestroy(bs));
+    if (t.ptr) t.retired.push_back(t.ptr);
+    t.ptr = p;
+    t.log = lg;
+  }
+  return t.ptr;
+}
+
+const fr_t* roots_table(Device& d, int half_log) {
+  if (half_log < 0) half_log = 0;
+  // At least 2^20 roots (order 2^21, 32 MiB) on first use.
+  return grow(d, d.roots, half_log, 20, 27, hfr{}, true, "ntt: order above 2^28");
+}
+
+const fr_t* coset_table(Device& d, int log) {
+  return grow(d, d.coset, log, 20, 28, hf::from_u64(5), false, "coset: length above 2^28");
+}
+
+}  // namespace pk

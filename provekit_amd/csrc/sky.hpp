@@ -1,0 +1,114 @@
+// Skyscraper-v2 2-to-1 compression over BN254 Fr on the VALU.
+//
+// Semantics (skyscraper/core/src/reference.rs:41-98; the fast form in
+// generic.rs:77-102 computes the same function): inputs are two arbitrary
+// 256-bit little-endian integers l, r taken mod p; 18 Feistel rounds
+//   r += F_i(l) + rc[i]; swap(l, r)
+// with F_i(x) = x^2 * 2^-256 (one Montgomery squaring) for rounds 0-5, 8, 9,
+// 12-17 and F_i = bar for rounds 6, 7, 10, 11; the output is (l + l_in) mod p,
+// canonical.  bar(x): the canonical value's 32 LE bytes rotated by 16, the
+// S-box applied to every byte, read back mod p.
+//
+// Values are kept in [0, 2p) between rounds; bar canonicalises its input and
+// the output is canonical, so any representation choice gives the same bytes.
+#pragma once
+#include "fr.hpp"
+
+namespace pk {
+
+// Round constants rc[0..17] as integers < p, 8 LE u32 limbs each (the
+// Skyscraper BN254 t = 1 instance; skyscraper/core/src/constants.rs:30-49).
+__constant__ static const uint32_t kSkyRC[18][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {0x270bd744u, 0x903c4324u, 0x08a7d269u, 0x873125f7u, 0x06c83855u, 0x081dd279u, 0xea6d7667u, 0x276b1823u},
+    {0x4b378d71u, 0x7ac8edbbu, 0xd99e2cb7u, 0xe29d79f3u, 0x4c1a5a18u, 0x75141791u, 0x58a484a6u, 0x0cf02bd7u},
+    {0x69e5bc36u, 0xfa7adc67u, 0x7cca387du, 0x1c3f8e29u, 0x63481db0u, 0x0eb7730du, 0x18ede544u, 0x25b0e03fu},
+    {0x2f03cfb7u, 0x57847e65u, 0x68873404u, 0x33440b96u, 0x49af80bcu, 0x955a32e8u, 0xbe14ae70u, 0x002882fcu},
+    {0x6257d4d7u, 0x97923139u, 0x1b37d3c1u, 0x29989c3eu, 0x7f1277bau, 0x12ef02b4u, 0x1e2b7a9cu, 0x039ad857u},
+    {0xabbb7887u, 0xb5b48465u, 0xe6ba2d2bu, 0xa72a6bc5u, 0x712f7b29u, 0x4cd48043u, 0x0fc1fc1au, 0x1142d541u},
+    {0x059075d3u, 0x7ab2c156u, 0x047999b2u, 0x17cb3594u, 0x98f289f7u, 0x44f2c935u, 0x69bc0becu, 0x1d78439fu},
+    {0x138b8edbu, 0x05d7a965u, 0xd55c48b1u, 0x36ef35a3u, 0xac6f1628u, 0x8ddfb8a1u, 0x08f4ff82u, 0x258588a5u},
+    {0xfccb49e9u, 0x1596fb9au, 0x9a09a95bu, 0x9a7367d6u, 0x84e4c157u, 0x9bc43f69u, 0xd2f514feu, 0x13087879u},
+    {0x3b4109fau, 0x295ccd23u, 0xed868012u, 0xe1d72f89u, 0x4bc88a8eu, 0x2e9e1eeau, 0x98c45232u, 0x17dadee8u},
+    {0xaa1f486fu, 0x9a8590b4u, 0x30e9130eu, 0xb75834b4u, 0x34d5de31u, 0xb8e90b10u, 0x46e7f4a6u, 0x295c6d15u},
+    {0x4c6eb892u, 0x850adcb7u, 0x05b92fc3u, 0x07699ef3u, 0xa1720f2du, 0x4ef96a2bu, 0x1d3ed446u, 0x1288ca0eu},
+    {0x49d1b5eeu, 0x01960f93u, 0x69371c69u, 0x8ccad307u, 0x91c98662u, 0xe5c81e89u, 0x1ae023f3u, 0x17563b4du},
+    {0x76b32917u, 0x6ba01e94u, 0xdd977bc9u, 0xa1cb0a3au, 0x5815f030u, 0x86815a94u, 0xe91a1eeau, 0x2869043bu},
+    {0x5511d976u, 0x81776c88u, 0x47f414e7u, 0x7475d34fu, 0x095d96cfu, 0x5d090056u, 0xff59e79au, 0x14941f0au},
+    {0x8fc8c034u, 0xbc40b4fdu, 0xcce4fd48u, 0xbb7142c3u, 0x8a39005au, 0x31835675u, 0x90f4379fu, 0x1ce337a1u},
+    {0, 0, 0, 0, 0, 0, 0, 0},
+};
+
+// 4p (< 2^256), for reducing arbitrary 256-bit integers.
+__device__ __forceinline__ uint32_t p4_limb(int i) {
+  switch (i) {
+    case 0: return 0xc0000004u; case 1: return 0x0f87d64fu; case 2: return 0xe6e5c245u;
+    case 3: return 0xa0cfa121u; case 4: return 0x06056174u; case 5: return 0xe14116dau;
+    case 6: return 0x84c680a6u; default: return 0xc19139cbu;
+  }
+}
+__device__ __forceinline__ void cond_sub_4p(fr_t& a) {
+  fr_t d;
+  uint32_t br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) d.l[i] = subb(a.l[i], p4_limb(i), br);
+#pragma unroll
+  for (int i = 0; i < 8; i++) a.l[i] = br ? a.l[i] : d.l[i];
+}
+// any 256-bit integer (< 2^256 < 6p) -> [0, 2p)
+__device__ __forceinline__ void reduce_any(fr_t& a) {
+  cond_sub_4p(a);
+  fr_reduce2p(a);
+}
+
+// Byte S-box on the four bytes of a word: rotl1(v ^ (rotl1(~v) & rotl2(v) & rotl3(v))).
+__device__ __forceinline__ uint32_t sbox4(uint32_t v) {
+  const uint32_t nv = ~v;
+  const uint32_t r1 = ((nv & 0x7f7f7f7fu) << 1) | ((nv >> 7) & 0x01010101u);
+  const uint32_t r2 = ((v & 0x3f3f3f3fu) << 2) | ((v >> 6) & 0x03030303u);
+  const uint32_t r3 = ((v & 0x1f1f1f1fu) << 3) | ((v >> 5) & 0x07070707u);
+  const uint32_t t = v ^ (r1 & r2 & r3);
+  return ((t & 0x7f7f7f7fu) << 1) | ((t >> 7) & 0x01010101u);
+}
+
+// bar on a lazy value; lazy result.
+__device__ __forceinline__ fr_t sky_bar(fr_t x) {
+  fr_canon(x);
+  fr_t y;
+#pragma unroll
+  for (int i = 0; i < 8; i++) y.l[i] = sbox4(x.l[(i + 4) & 7]);
+  reduce_any(y);
+  return y;
+}
+
+__device__ __forceinline__ fr_t sky_rc(int round) {
+  fr_t c;
+#pragma unroll
+  for (int i = 0; i < 8; i++) c.l[i] = kSkyRC[round][i];
+  return c;
+}
+
+// compress(l, r) for arbitrary 256-bit inputs; canonical output.
+__device__ __forceinline__ fr_t sky_compress(fr_t l, fr_t r) {
+  reduce_any(l);
+  reduce_any(r);
+  const fr_t t = l;
+#pragma unroll 1
+  for (int round = 0; round < 18; round++) {
+    const bool use_bar = (round == 6) | (round == 7) | (round == 10) | (round == 11);
+    fr_t f;
+    if (use_bar)
+      f = sky_bar(l);
+    else
+      f = fr_sqr(l);
+    fr_t s = fr_add(r, f);
+    if (round != 0 && round != 17) s = fr_add(s, sky_rc(round));
+    r = l;
+    l = s;
+  }
+  fr_t o = fr_add(l, t);
+  fr_canon(o);
+  return o;
+}
+
+}  // namespace pk

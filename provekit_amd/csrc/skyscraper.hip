@@ -1,0 +1,107 @@
+// Skyscraper hashing on gfx950: batch 2-to-1 compression, leaf hashing by
+// left fold (SkyscraperHashEngine::hash_many,
+// provekit/common/src/skyscraper/whir.rs:42-100) and binary Merkle trees.
+//
+// Every compression is ~5.6k VALU instructions (14 Montgomery squarings, 4
+// byte-S-box layers, lazy reductions), so these kernels are VALU-bound; one
+// lane owns one message, which keeps every lane busy with no cross-lane
+// traffic.  The upper tree levels are latency-bound (a compression takes
+// ~9 us on one wave), so the levels below 2048 nodes run in one workgroup.
+#include "kernels.hpp"
+#include "sky.hpp"
+
+namespace pk {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_compress_many(const fr_t* pairs, fr_t* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fr_t l = fr_load(pairs + 2 * i), r = fr_load(pairs + 2 * i + 1);
+  fr_store(out + i, sky_compress(l, r));
+}
+
+// One lane per message of `chunks` words: acc = m0; acc = compress(acc, m_k).
+__global__ __launch_bounds__(256) void k_hash_fold(const fr_t* in, uint64_t chunks, uint64_t count,
+                                                   fr_t* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const fr_t* m = in + i * chunks;
+  fr_t acc = fr_load(m);
+  fr_t nxt = fr_load(m + 1);
+  for (uint64_t k = 1; k < chunks; k++) {
+    const fr_t cur = nxt;
+    if (k + 1 < chunks) nxt = fr_load(m + k + 1);  // next word in flight
+    acc = sky_compress(acc, cur);
+  }
+  fr_store(out + i, acc);
+}
+
+__global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out, uint64_t n_out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  fr_store(out + i, sky_compress(fr_load(in + 2 * i), fr_load(in + 2 * i + 1)));
+}
+
+// Levels from n_in (<= 2048) nodes down to the root in one workgroup; level
+// outputs go to global memory, ordered by barriers.
+__global__ __launch_bounds__(1024) void k_merkle_top(fr_t* nodes, uint64_t n_in) {
+  fr_t* lvl = nodes;
+  uint64_t n = n_in;
+  while (n > 1) {
+    const uint64_t h = n / 2;
+    fr_t* up = lvl + n;
+    for (uint64_t i = threadIdx.x; i < h; i += blockDim.x)
+      fr_store(up + i, sky_compress(fr_load(lvl + 2 * i), fr_load(lvl + 2 * i + 1)));
+    __threadfence_block();
+    __syncthreads();
+    lvl = up;
+    n = h;
+  }
+}
+
+__global__ void k_copy(const fr_t* in, fr_t* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fr_store(out + i, fr_load(in + i));
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void sky_compress_many_dev(const fr_t* pairs, fr_t* out, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_compress_many<<<blocks_for(n, 256), 256, 0, s>>>(pairs, out, n);
+  PK_HIP(hipGetLastError());
+}
+
+void sky_hash_many_dev(uint64_t chunks, const fr_t* in, uint64_t count, fr_t* out, hipStream_t s) {
+  if (!count) return;
+  if (chunks == 1) {
+    k_copy<<<blocks_for(count, 256), 256, 0, s>>>(in, out, count);
+  } else {
+    k_hash_fold<<<blocks_for(count, 256), 256, 0, s>>>(in, chunks, count, out);
+  }
+  PK_HIP(hipGetLastError());
+}
+
+void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s) {
+  PK_REQUIRE(is_pow2(n), PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
+  if (leaves != nodes) {
+    k_copy<<<blocks_for(n, 256), 256, 0, s>>>(leaves, nodes, n);
+    PK_HIP(hipGetLastError());
+  }
+  fr_t* lvl = nodes;
+  while (n > 2048) {
+    k_merkle_level<<<blocks_for(n / 2, 256), 256, 0, s>>>(lvl, lvl + n, n / 2);
+    PK_HIP(hipGetLastError());
+    lvl += n;
+    n /= 2;
+  }
+  if (n > 1) {
+    k_merkle_top<<<1, 1024, 0, s>>>(lvl, n);
+    PK_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace pk

@@ -1,0 +1,101 @@
+"""GPU parity: ntt_nr / intt_rn / RSFr on the HIP kernels, bit-exact against
+the oracle (oracle/ntt.c) on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from provekit_amd.inputs import random_fr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def test_config1_forward_ntt_2p16(gpu, oracle):
+    # forward NTT of 2^16 uniform Fr, natural order in, bit-reversed out
+    v = random_fr(1 << 16, seed=1, stream="ntt-2^16")
+    say("ntt_nr 2^16")
+    a, b = v.copy(), v.copy()
+    gpu.ntt_nr(a, 1 << 16, 1)
+    oracle.ntt_nr(b, 1 << 16, 1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("k", list(range(0, 15)) + [18, 20])
+def test_ntt_sizes_and_round_trip(gpu, oracle, k):
+    n = 1 << k
+    v = random_fr(n, seed=100 + k, stream="ntt")
+    say(f"ntt_nr 2^{k}")
+    a, b = v.copy(), v.copy()
+    gpu.ntt_nr(a, n, 1)
+    oracle.ntt_nr(b, n, 1)
+    assert np.array_equal(a, b)
+    gpu.intt_rn(a)  # round_trip_ntt (ntt/src/ntt.rs:415-430)
+    assert np.array_equal(a, v)
+
+
+@pytest.mark.parametrize("rows_log,cols_log", [(0, 3), (3, 2), (10, 1), (12, 3), (6, 6), (16, 2)])
+def test_interleaved(gpu, oracle, rows_log, cols_log):
+    n = 1 << (rows_log + cols_log)
+    v = random_fr(n, seed=rows_log * 100 + cols_log, stream="interleaved")
+    say(f"interleaved rows=2^{rows_log} cols=2^{cols_log}")
+    a, b = v.copy(), v.copy()
+    gpu.ntt_nr(a, 1 << rows_log, 1)
+    oracle.ntt_nr(b, 1 << rows_log, 1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("k,groups", [(10, 2), (12, 4), (16, 8), (20, 4), (14, 1 << 14)])
+def test_num_groups(gpu, oracle, k, groups):
+    n = 1 << k
+    v = random_fr(n, seed=k * 7 + groups, stream="groups")
+    say(f"ntt_nr 2^{k} num_groups={groups}")
+    a, b = v.copy(), v.copy()
+    gpu.ntt_nr(a, n, groups)
+    oracle.ntt_nr(b, n, groups)
+    assert np.array_equal(a, b)
+
+
+def test_ntt_errors(gpu):
+    import provekit_amd as pk
+    v = random_fr(12, seed=1)
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        gpu.ntt_nr(v, 4, 5)
+    assert "divisible by num_groups" in str(e.value)
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        gpu.ntt_nr(v, 12, 1)
+    assert e.value.code == pk.ERR_NOT_POW2
+    empty = np.zeros((0, 4), np.uint64)
+    gpu.ntt_nr(empty, 0, 1)  # ntt_empty (ntt.rs:507-512)
+
+
+@pytest.mark.parametrize("log_msg,log_extra,M,log_mask", [
+    (0, 0, 1, 0), (1, 1, 1, 0), (2, 3, 3, 1), (4, 2, 4, 3), (3, 0, 2, 2), (4, 3, 1, 3), (2, 1, 4, 0),
+    (10, 2, 1, 0), (12, 1, 2, 4)])
+def test_rs_interleaved_encode(gpu, oracle, log_msg, log_extra, M, log_mask):
+    mlen, mask_len = 1 << log_msg, 1 << log_mask
+    codeword = (1 << (mlen + mask_len - 1).bit_length()) << log_extra
+    msgs = [random_fr(mlen, seed=r, stream=f"rs{log_msg}{log_extra}{M}{log_mask}") for r in range(M)]
+    masks = random_fr(M * mask_len, seed=99, stream="rsmask")
+    say(f"interleaved_encode msg=2^{log_msg} M={M} mask={mask_len} codeword={codeword}")
+    got = gpu.RSFr().interleaved_encode(msgs, masks, codeword)
+    exp = oracle.rs_interleaved_encode(msgs, masks, codeword)
+    assert np.array_equal(got, exp)
+
+
+def test_rs_no_masks_blowup(gpu, oracle):
+    # the commit shape: one message, no masks, blowup 4 (num_cosets = 4)
+    m = random_fr(1 << 14, seed=3, stream="rs-commit")
+    got = gpu.RSFr().interleaved_encode([m], None, 1 << 16)
+    exp = oracle.rs_interleaved_encode([m], None, 1 << 16)
+    assert np.array_equal(got, exp)
+
+
+def test_rs_points_generator_next_order(gpu, oracle):
+    rs = gpu.RSFr()
+    for cw in (1, 2, 64, 1 << 20):
+        idx = [0, 1, 2, 3, cw - 1, cw, 5 * cw + 3] if cw > 1 else [0, 1, 7]
+        assert np.array_equal(rs.evaluation_points(0, cw, idx), oracle.rs_evaluation_points(cw, idx))
+        assert np.array_equal(rs.generator(cw), oracle.root_of_unity(cw))
+    assert rs.next_order(1000) == 1024 and rs.next_order((1 << 28) + 1) is None

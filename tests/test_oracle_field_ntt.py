@@ -1,0 +1,215 @@
+"""Pins the oracle's field arithmetic and NTT / RSFr restatement against
+closed forms (Python big integers) and the reference's own test properties
+(ntt/src/ntt.rs:415-544, provekit/common/src/ntt.rs:120-180)."""
+import random
+
+import numpy as np
+import pytest
+
+from tests.util import P, R, RINV, limbs, mont, to_int, unmont
+
+pytestmark = []
+
+
+def rand_mont(rng, n):
+    return np.array([mont(rng.randrange(P)) for _ in range(n)], dtype=np.uint64).reshape(-1, 4)
+
+
+def test_fr_mul_closed_form(oracle):
+    rng = random.Random(1)
+    edge = [0, 1, P - 1, P - 2, (1 << 192) % P, (1 << 255) % P]
+    for _ in range(300):
+        a, b = rng.randrange(P), rng.randrange(P)
+        for x, y in ((a, b), (rng.choice(edge), b)):
+            out = to_int(oracle.fr_mul(limbs(x), limbs(y)))
+            assert out == x * y * RINV % P
+            assert out < P
+    # add / sub against the integers
+    for _ in range(100):
+        a, b = rng.randrange(P), rng.randrange(P)
+        assert to_int(oracle.fr_add(limbs(a), limbs(b))) == (a + b) % P
+        assert to_int(oracle.fr_sub(limbs(a), limbs(b))) == (a - b) % P
+
+
+def test_scalar_sqr_semantics(oracle):
+    # bn254_multiplier::scalar_sqr contract (scalar.rs:165-205): output below
+    # 2^256 - 2p and congruent to a^2 * 2^-256; inputs below 2p.
+    rng = random.Random(2)
+    out_max = (1 << 256) - 2 * P
+    for _ in range(500):
+        a = rng.randrange(2 * P)
+        s = to_int(oracle.scalar_sqr(limbs(a)))
+        assert s < out_max
+        assert s % P == a * a * RINV % P
+
+
+def test_root_of_unity(oracle):
+    # 5^((p-1)/2^28) is the 2^28-th root; get_root_of_unity(2^k) = it^(2^(28-k))
+    t = (P - 1) >> 28
+    g = pow(5, t, P)
+    assert g == 19103219067921713944291392827692070036145651957329286315305642004821462161904
+    assert pow(g, 1 << 28, P) == 1 and pow(g, 1 << 27, P) != 1
+    for k in (0, 1, 2, 5, 16, 22, 28):
+        w = unmont(oracle.root_of_unity(1 << k))
+        assert w == pow(g, 1 << (28 - k), P)
+
+
+def bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def brute_ntt(coeffs, N, omega):
+    """out[i] = P(omega^{bitrev(i)}) — the closed form of ntt_nr."""
+    bits = N.bit_length() - 1
+    out = []
+    for i in range(N):
+        x = pow(omega, bitrev(i, bits), P)
+        acc = 0
+        for c in reversed(coeffs):
+            acc = (acc * x + c) % P
+        out.append(acc)
+    return out
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 6, 8, 10])
+def test_ntt_nr_brute_force(oracle, k):
+    N = 1 << k
+    rng = random.Random(10 + k)
+    coeffs = [rng.randrange(P) for _ in range(N)]
+    v = np.array([mont(c) for c in coeffs], dtype=np.uint64)
+    oracle.ntt_nr(v, N, 1)
+    w = unmont(oracle.root_of_unity(N))
+    assert [unmont(x) for x in v] == brute_ntt(coeffs, N, w)
+
+
+def test_ntt_large_split_path_matches_small_path(oracle):
+    # 2^13 goes through the two large-stride loops (ntt.rs:189-219) then the
+    # cache path; check against evaluation at a few points.
+    N = 1 << 13
+    rng = random.Random(5)
+    coeffs = [rng.randrange(P) for _ in range(N)]
+    v = np.array([mont(c) for c in coeffs], dtype=np.uint64)
+    oracle.ntt_nr(v, N, 1)
+    w = unmont(oracle.root_of_unity(N))
+    for i in (0, 1, 2, 777, N - 1):
+        x = pow(w, bitrev(i, 13), P)
+        acc = 0
+        for c in reversed(coeffs):
+            acc = (acc * x + c) % P
+        assert unmont(v[i]) == acc
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5, 9, 12, 14])
+def test_round_trip(oracle, k):
+    # round_trip_ntt (ntt.rs:415-430)
+    rng = np.random.default_rng(k)
+    n = 1 << k
+    v = np.array([mont(int(x)) for x in rng.integers(0, 2**62, n)], dtype=np.uint64).reshape(-1, 4)
+    orig = v.copy()
+    oracle.ntt_nr(v, n, 1)
+    oracle.intt_rn(v)
+    assert np.array_equal(v, orig)
+
+
+@pytest.mark.parametrize("rows_log,cols_log", [(3, 0), (3, 2), (5, 1), (4, 4), (0, 3), (10, 2)])
+def test_interleaved(oracle, rows_log, cols_log):
+    # test_interleaved (ntt.rs:488-505): the interleaved transform of a
+    # rows x columns matrix equals per-column NTTs.
+    rows, cols = 1 << rows_log, 1 << cols_log
+    rng = random.Random(rows * 31 + cols)
+    mat = np.array([mont(rng.randrange(P)) for _ in range(rows * cols)], dtype=np.uint64)
+    ref = mat.reshape(rows, cols, 4).transpose(1, 0, 2).copy()
+    for c in range(cols):
+        col = np.ascontiguousarray(ref[c])
+        oracle.ntt_nr(col, rows, 1)
+        ref[c] = col
+    v = mat.copy()
+    oracle.ntt_nr(v, rows, 1)
+    assert np.array_equal(v.reshape(rows, cols, 4), ref.transpose(1, 0, 2))
+
+
+def test_roots_prefix_consistent(oracle):
+    # roots_direct_vs_extended (ntt.rs:515-522): a table of order 2^k is the
+    # prefix of the table of order 2^(k+1).
+    small, big = oracle.roots_table(1 << 10), oracle.roots_table(1 << 12)
+    assert np.array_equal(big[: small.shape[0]], small)
+    w = unmont(oracle.root_of_unity(1 << 12))
+    for j in (0, 1, 5, 1000, 2047):
+        assert unmont(big[j]) == pow(w, bitrev(j, 11), P)
+
+
+def test_ntt_empty_and_one(oracle):
+    v = np.zeros((0, 4), np.uint64)
+    oracle.ntt_nr(v, 0, 1)
+    one = np.array([mont(7)], dtype=np.uint64)
+    oracle.ntt_nr(one, 1, 1)
+    assert unmont(one[0]) == 7
+
+
+def test_reverse_order_round_trip(oracle):
+    v = np.arange(64 * 4, dtype=np.uint64).reshape(64, 4)
+    w = v.copy()
+    oracle.reverse_order(w)
+    assert not np.array_equal(v, w)
+    oracle.reverse_order(w)
+    assert np.array_equal(v, w)
+
+
+def brute_rs(messages, masks, codeword):
+    """RSFr::interleaved_encode closed form: out[i*M + r] = P_r(w^{bitrev(i)})
+    where P_r's coefficients are message r then its mask entries
+    masks[c*M + r] (row-major interleaved masks)."""
+    M = len(messages)
+    mlen = len(messages[0])
+    mask_len = len(masks) // M
+    polys = []
+    for r in range(M):
+        c = list(messages[r]) + [masks[k * M + r] for k in range(mask_len)]
+        polys.append(c)
+    w = pow(pow(5, (P - 1) >> 28, P), (1 << 28) // codeword, P)
+    bits = codeword.bit_length() - 1
+    out = []
+    for i in range(codeword):
+        x = pow(w, bitrev(i, bits), P)
+        for r in range(M):
+            acc = 0
+            for c in reversed(polys[r]):
+                acc = (acc * x + c) % P
+            out.append(acc)
+    return out
+
+
+@pytest.mark.parametrize("log_msg,log_extra,M,log_mask", [
+    (0, 0, 1, 0), (1, 1, 1, 0), (2, 3, 3, 1), (4, 2, 4, 3), (3, 0, 2, 2), (4, 3, 1, 3), (2, 1, 4, 0)])
+def test_rs_interleaved_encode_closed_form(oracle, log_msg, log_extra, M, log_mask):
+    # the parameter space of interleaved_encode_matches_whir_reference
+    # (provekit/common/src/ntt.rs:120-129), checked against the closed form
+    # instead of whir's NttEngine (not vendored).
+    rng = random.Random(log_msg * 1000 + log_extra * 100 + M * 10 + log_mask)
+    mlen, mask_len = 1 << log_msg, 1 << log_mask
+    codeword = (1 << (mlen + mask_len - 1).bit_length()) << log_extra
+    msgs = [[rng.randrange(P) for _ in range(mlen)] for _ in range(M)]
+    masks = [rng.randrange(P) for _ in range(M * mask_len)]
+    got = oracle.rs_interleaved_encode([np.array([mont(x) for x in m]) for m in msgs],
+                                       np.array([mont(x) for x in masks]), codeword)
+    assert [unmont(x) for x in got] == brute_rs(msgs, masks, codeword)
+
+
+def test_rs_regressions(oracle, kats):
+    # provekit/common/proptest-regressions/ntt.txt: elements equal to 2^192
+    assert len(kats["ntt_regressions_raw"]) == 2
+    v = 1 << 192
+    got = oracle.rs_interleaved_encode([np.array([mont(0), mont(v)])], np.array([mont(0)]), 8)
+    assert [unmont(x) for x in got] == brute_rs([[0, v]], [0], 8)
+    got = oracle.rs_interleaved_encode([np.zeros((1, 4), np.uint64), np.zeros((1, 4), np.uint64)],
+                                       np.array([mont(0), mont(v), mont(0), mont(0)]), 4)
+    assert [unmont(x) for x in got] == brute_rs([[0], [0]], [0, v, 0, 0], 4)
+
+
+def test_rs_evaluation_points(oracle):
+    w = pow(pow(5, (P - 1) >> 28, P), (1 << 28) // 64, P)
+    idx = [0, 1, 2, 63, 64, 100]
+    got = oracle.rs_evaluation_points(64, idx)
+    assert [unmont(x) for x in got] == [pow(w, bitrev(i % 64, 6), P) for i in idx]
+    assert [unmont(x) for x in oracle.rs_evaluation_points(1, [0, 5])] == [1, 1]
+    assert oracle.rs_next_order(5) == 8 and oracle.rs_next_order((1 << 28) + 1) is None
