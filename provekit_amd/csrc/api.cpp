@@ -68,9 +68,9 @@ void ntt_nr(Fr* values, size_t len, size_t codeword_size, size_t num_groups) {
   REQ(num_groups >= 1 && len % num_groups == 0, pk::PK_ERR_SIZE,
       "values.len() must be divisible by num_groups");
   REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
-  REQ(pk::is_pow2(len) && len % codeword_size == 0 && pk::is_pow2(num_groups) &&
-          num_groups <= codeword_size,
-      pk::PK_ERR_SIZE, "ntt_nr: len must be a power-of-two multiple of codeword_size");
+  REQ(len % codeword_size == 0 && pk::is_pow2(num_groups) && num_groups <= codeword_size,
+      pk::PK_ERR_SIZE,
+      "ntt_nr: len must be a multiple of codeword_size and num_groups a power of two up to it");
   hipStream_t s = pk::thread_stream();
   pk::Device& d = pk::device_current();
   Staged b(sizeof(fr_t) * len, s);
@@ -289,9 +289,9 @@ int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size
     REQ(num_groups >= 1 && len % num_groups == 0, pk::PK_ERR_SIZE,
         "values.len() must be divisible by num_groups");
     REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
-    REQ(pk::is_pow2(len) && len % codeword_size == 0 && pk::is_pow2(num_groups) &&
-            num_groups <= codeword_size,
-        pk::PK_ERR_SIZE, "ntt_nr: len must be a power-of-two multiple of codeword_size");
+    REQ(len % codeword_size == 0 && pk::is_pow2(num_groups) && num_groups <= codeword_size,
+        pk::PK_ERR_SIZE,
+        "ntt_nr: len must be a multiple of codeword_size and num_groups a power of two up to it");
     pk::ntt_nr_dev(pk::device_current(), reinterpret_cast<fr_t*>(values), len, codeword_size,
                    num_groups, stream_or_thread(stream));
   });

@@ -80,7 +80,8 @@ struct PassArgs {
   const fr_t* src;
   const fr_t* rr;
   uint64_t chunk;
-  int L, s0, r, jb, canon;
+  uint64_t Q;  // low stride: 2^q * M elements below the window (M = interleave width)
+  int r, jb, canon;
 };
 
 template <int TL>
@@ -90,6 +91,11 @@ struct PassShape {
   static constexpr int BPT = BF / T;
 };
 
+// Flat element index of codeword position i, column c: i * M + c.  A pass
+// over stages [s0, s0 + r) splits i = (h, w, i_lo) with w the r window bits
+// and i_lo the q bits below; the element sits at
+//   h * (2^r * Q) + w * Q + l,   l = i_lo * M + c in [0, Q).
+// A tile is one h, all 2^r windows, and 2^jb consecutive l (2^jb divides Q).
 template <int TL>
 __global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
   constexpr int T = PassShape<TL>::T;
@@ -97,17 +103,17 @@ __global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
   constexpr int N = 1 << TL;
   __shared__ fr_t lds[N];
   const int tid = threadIdx.x;
-  const int q = a.L - a.s0 - a.r;
   const int jb = a.jb;
-  const uint64_t tiles_per_h = (1ull << q) >> jb;
+  const uint64_t Q = a.Q;
+  const uint64_t tiles_per_h = Q >> jb;
   const uint64_t tile = blockIdx.x;
   const uint64_t h = tile / tiles_per_h;
   const uint64_t j0 = (tile % tiles_per_h) << jb;
-  const uint64_t base = (h << (a.L - a.s0)) | j0;
+  const uint64_t base = h * (Q << a.r) + j0;
   const uint32_t jmask = (1u << jb) - 1;
 
   for (int e = tid; e < N; e += T) {
-    const uint64_t g = base + ((uint64_t)(e >> jb) << q) + (e & jmask);
+    const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
     fr_t v = a.src ? fr_load(a.src + (g % a.chunk)) : fr_load(a.data + g);
     lds[e] = v;
   }
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
   }
 
   for (int e = tid; e < N; e += T) {
-    const uint64_t g = base + ((uint64_t)(e >> jb) << q) + (e & jmask);
+    const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
     fr_t v = lds[e];
     if (a.canon) fr_canon(v);
     fr_store(a.data + g, v);
@@ -183,10 +189,12 @@ struct NttPass {
   int s0, r, jb;
 };
 
-// Pass plan for stages [s_begin, s_end) of a length-2^L array.  The last pass
-// takes as many stages as fit a full tile; earlier passes split the rest
-// evenly, each keeping at least 2^kMinRun consecutive elements per run.
-static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end) {
+// Pass plan for stages [s_begin, s_end) of codeword index bits (L = s_end)
+// with interleave width M.  The last pass takes as many stages as fit a full
+// tile; earlier passes split the rest evenly, each with at least 2^kMinRun
+// consecutive elements per run where the layout allows.  jb is capped by the
+// power of two dividing the low stride 2^q * M.
+static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_twos) {
   constexpr int kTileLog = 11, kMinRun = 2;
   std::vector<NttPass> out;
   int rem = s_end - s_begin;
@@ -197,24 +205,30 @@ static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end) {
   int s0 = s_begin;
   for (int i = 0; i < nfirst; i++) {
     int r = rest / nfirst + (i < rest % nfirst ? 1 : 0);
-    int q = L - s0 - r;
-    int jb = q < (kTileLog - r) ? q : (kTileLog - r);
+    int q2 = (L - s0 - r) + m_twos;
+    int jb = q2 < (kTileLog - r) ? q2 : (kTileLog - r);
     out.push_back({s0, r, jb});
     s0 += r;
   }
-  int q = L - s_end;
-  int jb = q < (kTileLog - r_fin) ? q : (kTileLog - r_fin);
+  int q2 = (L - s_end) + m_twos;
+  int jb = q2 < (kTileLog - r_fin) ? q2 : (kTileLog - r_fin);
   out.push_back({s0, r_fin, jb});
   return out;
 }
 
 void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
                 hipStream_t s, const fr_t* src, uint64_t chunk) {
-  PK_REQUIRE(is_pow2(len), PK_ERR_NOT_POW2, "ntt: length must be a power of two");
-  const int L = log2_exact(len);
+  PK_REQUIRE(codeword >= 1 && is_pow2(codeword) && len % codeword == 0, PK_ERR_SIZE,
+             "ntt: length must be a multiple of the power-of-two codeword size");
+  PK_REQUIRE(is_pow2(num_groups) && num_groups <= codeword, PK_ERR_SIZE,
+             "ntt: num_groups must be a power of two up to the codeword size");
+  const uint64_t M = len / codeword;
+  int m_twos = 0;
+  while (((M >> m_twos) & 1) == 0) m_twos++;
+  const int L = log2_exact(codeword);
   const int s_begin = log2_exact(num_groups);
-  const int s_end = codeword > 1 ? log2_exact(codeword) : 0;
-  std::vector<NttPass> plan = plan_passes(L, s_begin, s_end);
+  const int s_end = L;
+  std::vector<NttPass> plan = plan_passes(L, s_begin, s_end, m_twos);
   if (plan.empty()) {
     if (src) {
       k_replicate<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(data, src, chunk, len);
@@ -229,8 +243,7 @@ void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t
     a.src = (i == 0) ? src : nullptr;
     a.chunk = chunk;
     a.rr = rr;
-    a.L = L;
-    a.s0 = plan[i].s0;
+    a.Q = (1ull << (L - plan[i].s0 - plan[i].r)) * M;
     a.r = plan[i].r;
     a.jb = plan[i].jb;
     a.canon = (i + 1 == plan.size());
