@@ -117,6 +117,30 @@ int pkgpu_skyscraper_hash_many_device(size_t size, const void* input, size_t cou
  * last (nodes may equal leaves). */
 int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, void* stream);
 
+/* ------------------------------------------------------------ Poseidon2 */
+/* poseidon2::poseidon2_permutation (poseidon2/src/permutation.rs:174-176) on
+ * n states of 4 Fr (4n elements, row-major). */
+int pkgpu_poseidon2_permutation_many(const uint64_t* states, size_t n, uint64_t* out);
+/* poseidon2::poseidon2_hash (poseidon2/src/hash.rs:25-55) on one message. */
+int pkgpu_poseidon2_hash(const uint64_t* inputs, size_t len, uint64_t out[4]);
+/* poseidon2_hash on `count` messages of `len` elements each (row-major). */
+int pkgpu_poseidon2_hash_many(const uint64_t* inputs, size_t len, size_t count, uint64_t* out);
+int pkgpu_poseidon2_hash_many_device(const uint64_t* inputs, size_t len, size_t count,
+                                     uint64_t* out, void* stream);
+/* Merkle tree with poseidon2_hash([left, right]) per node, as
+ * compute_merkle_root (playground/passport-input-gen/src/commitment.rs:225-237)
+ * walks it: n leaves (power of two); nodes (2n-1 Fr, may be NULL) level by
+ * level, root last. */
+int pkgpu_poseidon2_merkle(const uint64_t* leaves, size_t n, uint64_t* nodes, uint64_t root[4]);
+int pkgpu_poseidon2_merkle_device(const uint64_t* leaves, size_t n, uint64_t* nodes,
+                                  void* stream);
+/* The device tree's level launches recorded once into a hipGraph bound to
+ * these buffers; replay with pkgpu_graph_launch, free with pkgpu_graph_destroy. */
+int pkgpu_poseidon2_merkle_graph(const uint64_t* leaves, size_t n, uint64_t* nodes,
+                                 void** graph);
+int pkgpu_graph_launch(void* graph, void* stream);
+int pkgpu_graph_destroy(void* graph);
+
 /* ------------------------------------------------------------ commit */
 /* WHIR-style commitment of one polynomial of n coefficients (n, blowup powers
  * of two): RS-encode to n*blowup elements (RSFr order; coset != 0: on 5*H),

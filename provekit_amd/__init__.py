@@ -65,6 +65,15 @@ SIGNATURES = {
     "pkgpu_skyscraper_hash_many": [_sz, _vp, _sz, _vp, _sz],
     "pkgpu_skyscraper_hash_many_device": [_sz, _vp, _sz, _vp, _vp],
     "pkgpu_skyscraper_merkle_device": [_vp, _sz, _vp, _vp],
+    "pkgpu_poseidon2_permutation_many": [_vp, _sz, _vp],
+    "pkgpu_poseidon2_hash": [_vp, _sz, _vp],
+    "pkgpu_poseidon2_hash_many": [_vp, _sz, _sz, _vp],
+    "pkgpu_poseidon2_hash_many_device": [_vp, _sz, _sz, _vp, _vp],
+    "pkgpu_poseidon2_merkle": [_vp, _sz, _vp, _vp],
+    "pkgpu_poseidon2_merkle_device": [_vp, _sz, _vp, _vp],
+    "pkgpu_poseidon2_merkle_graph": [_vp, _sz, _vp, ctypes.POINTER(_vp)],
+    "pkgpu_graph_launch": [_vp, _vp],
+    "pkgpu_graph_destroy": [_vp],
     "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
     "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
 }
@@ -232,6 +241,76 @@ class SkyscraperHashEngine:
         _call("pkgpu_skyscraper_hash_many", size, _ptr(m) if len(m) else None, len(m),
               _ptr(out) if len(out) else None, count)
         return out.tobytes()
+
+
+# ----------------------------------------------------------------- Poseidon2
+def poseidon2_permutation(state) -> np.ndarray:
+    """poseidon2_permutation (poseidon2/src/permutation.rs:174): (4, 4) -> (4, 4);
+    also accepts (n, 4, 4) for n states."""
+    _ensure_init()
+    s = np.ascontiguousarray(state, dtype=np.uint64)
+    single = s.ndim == 2
+    s = s.reshape(-1, 4, 4)
+    out = np.zeros_like(s)
+    _call("pkgpu_poseidon2_permutation_many", _ptr(s), s.shape[0], _ptr(out))
+    return out[0] if single else out
+
+
+def poseidon2_hash(inputs) -> np.ndarray:
+    """poseidon2_hash (poseidon2/src/hash.rs:25)."""
+    _ensure_init()
+    x = _fr_array(inputs) if len(inputs) else np.zeros((0, 4), np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    _call("pkgpu_poseidon2_hash", _ptr(x) if x.size else None, x.shape[0], _ptr(out))
+    return out
+
+
+def poseidon2_hash_many(messages: np.ndarray) -> np.ndarray:
+    """(count, len, 4) messages -> (count, 4) digests."""
+    _ensure_init()
+    m = np.ascontiguousarray(messages, dtype=np.uint64)
+    count, ln = m.shape[0], m.shape[1]
+    out = np.zeros((count, 4), dtype=np.uint64)
+    _call("pkgpu_poseidon2_hash_many", _ptr(m) if m.size else None, ln, count, _ptr(out))
+    return out
+
+
+def poseidon2_merkle(leaves: np.ndarray, want_nodes: bool = True):
+    """Poseidon2 2-to-1 Merkle tree; returns (root (4,), nodes (2n-1, 4) or None)."""
+    _ensure_init()
+    x = _fr_array(leaves)
+    n = x.shape[0]
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.uint64) if want_nodes else None
+    root = np.zeros(4, dtype=np.uint64)
+    _call("pkgpu_poseidon2_merkle", _ptr(x), n, _ptr(nodes) if nodes is not None else None, _ptr(root))
+    return root, nodes
+
+
+class Graph:
+    """An executable hipGraph made by one of the *_graph entry points."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream=None) -> None:
+        _call("pkgpu_graph_launch", self.handle, stream)
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().pkgpu_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def poseidon2_merkle_graph(leaves: "DeviceBuffer", n: int, nodes: "DeviceBuffer") -> Graph:
+    h = ctypes.c_void_p()
+    _call("pkgpu_poseidon2_merkle_graph", leaves.ptr, n, nodes.ptr, ctypes.byref(h))
+    return Graph(h.value)
 
 
 # ----------------------------------------------------------------- commit

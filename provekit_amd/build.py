@@ -25,6 +25,7 @@ SOURCES = [
     "ntt.hip",
     "skyscraper.hip",
     "commit.hip",
+    "poseidon2.hip",
 ]
 
 ARCH = "gfx950"

@@ -415,6 +415,85 @@ int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, vo
   });
 }
 
+int pkgpu_poseidon2_permutation_many(const uint64_t* states, size_t n, uint64_t* out) {
+  return guarded([&] {
+    if (!n) return;
+    REQ(states && out, pk::PK_ERR_ARG, "poseidon2_permutation: null pointer");
+    hipStream_t s = pk::thread_stream();
+    Staged di(sizeof(fr_t) * 4 * n, s), dout(sizeof(fr_t) * 4 * n, s);
+    h2d(di.get<void>(), states, sizeof(fr_t) * 4 * n, s);
+    pk::p2_permutation_many_dev(pk::device_current(), di.get<fr_t>(), dout.get<fr_t>(), n, s);
+    d2h(out, dout.get<void>(), sizeof(fr_t) * 4 * n, s);
+    sync(s);
+  });
+}
+int pkgpu_poseidon2_hash_many(const uint64_t* inputs, size_t len, size_t count, uint64_t* out) {
+  return guarded([&] {
+    if (!count) return;
+    REQ(out && (len == 0 || inputs), pk::PK_ERR_ARG, "poseidon2_hash: null pointer");
+    hipStream_t s = pk::thread_stream();
+    Staged di(sizeof(fr_t) * len * count, s), dout(sizeof(fr_t) * count, s);
+    h2d(di.get<void>(), inputs, sizeof(fr_t) * len * count, s);
+    pk::p2_hash_many_dev(pk::device_current(), di.get<fr_t>(), len, count, dout.get<fr_t>(), s);
+    d2h(out, dout.get<void>(), sizeof(fr_t) * count, s);
+    sync(s);
+  });
+}
+int pkgpu_poseidon2_hash(const uint64_t* inputs, size_t len, uint64_t out[4]) {
+  return pkgpu_poseidon2_hash_many(inputs, len, 1, out);
+}
+int pkgpu_poseidon2_hash_many_device(const uint64_t* inputs, size_t len, size_t count,
+                                     uint64_t* out, void* stream) {
+  return guarded([&] {
+    if (!count) return;
+    REQ(out && (len == 0 || inputs), pk::PK_ERR_ARG, "poseidon2_hash: null pointer");
+    pk::p2_hash_many_dev(pk::device_current(), reinterpret_cast<const fr_t*>(inputs), len, count,
+                         reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
+int pkgpu_poseidon2_merkle(const uint64_t* leaves, size_t n, uint64_t* nodes, uint64_t root[4]) {
+  return guarded([&] {
+    REQ(leaves && root, pk::PK_ERR_ARG, "poseidon2_merkle: null pointer");
+    REQ(pk::is_pow2(n), pk::PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
+    hipStream_t s = pk::thread_stream();
+    const size_t total = 2 * n - 1;
+    Staged dn(sizeof(fr_t) * total, s);
+    h2d(dn.get<void>(), leaves, sizeof(fr_t) * n, s);
+    pk::p2_merkle_dev(pk::device_current(), dn.get<fr_t>(), n, dn.get<fr_t>(), s);
+    d2h(root, dn.get<fr_t>() + (total - 1), 32, s);
+    if (nodes) d2h(nodes, dn.get<void>(), sizeof(fr_t) * total, s);
+    sync(s);
+  });
+}
+int pkgpu_poseidon2_merkle_device(const uint64_t* leaves, size_t n, uint64_t* nodes,
+                                  void* stream) {
+  return guarded([&] {
+    REQ(leaves && nodes, pk::PK_ERR_ARG, "poseidon2_merkle: null pointer");
+    pk::p2_merkle_dev(pk::device_current(), reinterpret_cast<const fr_t*>(leaves), n,
+                      reinterpret_cast<fr_t*>(nodes), stream_or_thread(stream));
+  });
+}
+int pkgpu_poseidon2_merkle_graph(const uint64_t* leaves, size_t n, uint64_t* nodes,
+                                 void** graph) {
+  return guarded([&] {
+    REQ(leaves && nodes && graph, pk::PK_ERR_ARG, "poseidon2_merkle_graph: null pointer");
+    REQ(pk::is_pow2(n), pk::PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
+    *graph = pk::p2_merkle_graph(pk::device_current(), reinterpret_cast<const fr_t*>(leaves), n,
+                                 reinterpret_cast<fr_t*>(nodes));
+  });
+}
+int pkgpu_graph_launch(void* graph, void* stream) {
+  return guarded([&] {
+    REQ(graph, pk::PK_ERR_STATE, "graph_launch: null graph");
+    PK_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), stream_or_thread(stream)));
+  });
+}
+int pkgpu_graph_destroy(void* graph) {
+  return guarded([&] {
+    if (graph) PK_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+  });
+}
+
 int pkgpu_commit(const uint64_t* coeffs, size_t n, size_t blowup, int coset, size_t leaf_elems,
                  uint64_t* codeword, uint8_t* nodes, uint8_t root[32]) {
   return guarded([&] {

@@ -163,7 +163,10 @@ __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
       if (j < k && q >= 1 && q < 8) macc(c01, c2, m[j], p_limb(q));
     }
     if (k < 8) {
-      m[k] = (uint32_t)c01 * PK_NP0;
+      // m = x * (-p^-1 mod 2^32) with -p^-1 = -(1 + 2^28): two full-rate ops
+      // instead of a quarter-rate v_mul_lo_u32
+      const uint32_t x = (uint32_t)c01;
+      m[k] = 0u - (x + (x << 28));
       macc(c01, c2, m[k], PK_P0);
     } else {
       r.l[k - 8] = (uint32_t)c01;

@@ -34,6 +34,17 @@ void sky_hash_many_dev(uint64_t chunks, const fr_t* in, uint64_t count, fr_t* ou
 // alias `leaves`), levels follow, root at nodes[2n-2].
 void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s);
 
+// ---- Poseidon2 (poseidon2.hip) ----
+// n states of 4 elements (row-major) -> permuted states.
+void p2_permutation_many_dev(Device& d, const fr_t* states, fr_t* out, uint64_t n, hipStream_t s);
+// poseidon2_hash of `count` messages of `len` elements each -> count digests.
+void p2_hash_many_dev(Device& d, const fr_t* in, uint64_t len, uint64_t count, fr_t* out,
+                      hipStream_t s);
+// 2-to-1 tree (hash([l, r])) over n leaves: nodes[0..n) leaves, root at 2n-2.
+void p2_merkle_dev(Device& d, const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s);
+// The same launches recorded once as an executable graph.
+hipGraphExec_t p2_merkle_graph(Device& d, const fr_t* leaves, uint64_t n, fr_t* nodes);
+
 // ---- commit pipeline (commit.hip) ----
 // One polynomial of n coefficients: RS encode at `blowup` (optionally on the
 // coset 5*H), leaves of `leaf_elems` consecutive codeword elements hashed by

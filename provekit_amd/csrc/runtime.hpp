@@ -67,7 +67,7 @@ struct Device {
   std::mutex tables_mtx;
   GrowTable roots;   // rr[j] = w^{bitrev(j)}, w of order 2^(log+1)
   GrowTable coset;   // g^j, g = 5 (multiplicative generator of Fr)
-  fr_t* p2_consts = nullptr;  // Poseidon2 round constants + diagonal (Montgomery)
+  bool p2_ready = false;  // Poseidon2 parameters uploaded to constant memory
 };
 
 Device& device_current();
@@ -76,7 +76,7 @@ Device& device_get(int id);
 // Tables holding at least 2^log entries (built on first use, then cached).
 const fr_t* roots_table(Device& d, int half_log);  // >= 2^half_log roots
 const fr_t* coset_table(Device& d, int log);       // >= 2^log powers of 5
-const fr_t* poseidon2_consts(Device& d);
+void poseidon2_consts(Device& d);
 
 // Stream-ordered scratch allocation freed at scope exit (on the same stream).
 struct Scratch {

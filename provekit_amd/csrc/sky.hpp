@@ -88,23 +88,22 @@ __device__ __forceinline__ fr_t sky_rc(int round) {
   return c;
 }
 
-// compress(l, r) for arbitrary 256-bit inputs; canonical output.
+// compress(l, r) for arbitrary 256-bit inputs; canonical output.  Written as
+// the reference's nine double rounds (ss / bb, reference.rs:49-78): two
+// Feistel rounds update r then l in place, so nothing is swapped.
 __device__ __forceinline__ fr_t sky_compress(fr_t l, fr_t r) {
   reduce_any(l);
   reduce_any(r);
   const fr_t t = l;
 #pragma unroll 1
-  for (int round = 0; round < 18; round++) {
-    const bool use_bar = (round == 6) | (round == 7) | (round == 10) | (round == 11);
-    fr_t f;
-    if (use_bar)
-      f = sky_bar(l);
-    else
-      f = fr_sqr(l);
-    fr_t s = fr_add(r, f);
-    if (round != 0 && round != 17) s = fr_add(s, sky_rc(round));
-    r = l;
-    l = s;
+  for (int dr = 0; dr < 9; dr++) {
+    const bool bb = (dr == 3) | (dr == 5);  // rounds 6, 7 and 10, 11
+    const fr_t f0 = bb ? sky_bar(l) : fr_sqr(l);
+    r = fr_add(r, f0);
+    if (dr != 0) r = fr_add(r, sky_rc(2 * dr));  // rc[0] = 0
+    const fr_t f1 = bb ? sky_bar(r) : fr_sqr(r);
+    l = fr_add(l, f1);
+    if (dr != 8) l = fr_add(l, sky_rc(2 * dr + 1));  // rc[17] = 0
   }
   fr_t o = fr_add(l, t);
   fr_canon(o);

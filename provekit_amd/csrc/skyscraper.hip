@@ -5,8 +5,9 @@
 // Every compression is ~5.6k VALU instructions (14 Montgomery squarings, 4
 // byte-S-box layers, lazy reductions), so these kernels are VALU-bound; one
 // lane owns one message, which keeps every lane busy with no cross-lane
-// traffic.  The upper tree levels are latency-bound (a compression takes
-// ~9 us on one wave), so the levels below 2048 nodes run in one workgroup.
+// traffic.  The upper tree levels are latency-bound (a compression is a long
+// dependent chain on one wave), so they run one wave per workgroup and the
+// last six levels run inside a single wave.
 #include "kernels.hpp"
 #include "sky.hpp"
 
@@ -43,20 +44,27 @@ __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out,
   fr_store(out + i, sky_compress(fr_load(in + 2 * i), fr_load(in + 2 * i + 1)));
 }
 
-// Levels from n_in (<= 2048) nodes down to the root in one workgroup; level
-// outputs go to global memory, ordered by barriers.
-__global__ __launch_bounds__(1024) void k_merkle_top(fr_t* nodes, uint64_t n_in) {
-  fr_t* lvl = nodes;
-  uint64_t n = n_in;
-  while (n > 1) {
-    const uint64_t h = n / 2;
-    fr_t* up = lvl + n;
-    for (uint64_t i = threadIdx.x; i < h; i += blockDim.x)
-      fr_store(up + i, sky_compress(fr_load(lvl + 2 * i), fr_load(lvl + 2 * i + 1)));
-    __threadfence_block();
+// The last levels, n_in <= 64 nodes down to the root, in one wave: the level
+// sits in LDS and each step halves the active lanes.  These levels are
+// latency-bound (one compression per level on the critical path), so a
+// single wave per level is as fast as any spread and needs no launches.
+__global__ __launch_bounds__(64) void k_merkle_tail(fr_t* nodes, uint64_t n_in) {
+  __shared__ fr_t lvl[64];
+  const int t = threadIdx.x;
+  if (t < (int)n_in) lvl[t] = fr_load(nodes + t);
+  __syncthreads();
+  uint64_t off = n_in;
+  for (uint64_t n = n_in; n > 1; n >>= 1) {
+    const uint64_t h = n >> 1;
+    fr_t v;
+    if ((uint64_t)t < h) v = sky_compress(lvl[2 * t], lvl[2 * t + 1]);
     __syncthreads();
-    lvl = up;
-    n = h;
+    if ((uint64_t)t < h) {
+      lvl[t] = v;
+      fr_store(nodes + off + t, v);
+    }
+    __syncthreads();
+    off += h;
   }
 }
 
@@ -92,14 +100,17 @@ void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s) 
     PK_HIP(hipGetLastError());
   }
   fr_t* lvl = nodes;
-  while (n > 2048) {
-    k_merkle_level<<<blocks_for(n / 2, 256), 256, 0, s>>>(lvl, lvl + n, n / 2);
+  while (n > 64) {
+    // one-wave workgroups: the narrow upper levels spread over CUs instead of
+    // sharing SIMDs
+    const unsigned t = n / 2 >= (1u << 16) ? 256 : 64;
+    k_merkle_level<<<blocks_for(n / 2, t), t, 0, s>>>(lvl, lvl + n, n / 2);
     PK_HIP(hipGetLastError());
     lvl += n;
     n /= 2;
   }
   if (n > 1) {
-    k_merkle_top<<<1, 1024, 0, s>>>(lvl, n);
+    k_merkle_tail<<<1, 64, 0, s>>>(lvl, n);
     PK_HIP(hipGetLastError());
   }
 }
