@@ -141,6 +141,53 @@ int pkgpu_poseidon2_merkle_graph(const uint64_t* leaves, size_t n, uint64_t* nod
 int pkgpu_graph_launch(void* graph, void* stream);
 int pkgpu_graph_destroy(void* graph);
 
+/* ------------------------------------------------------------ R1CS / sumcheck */
+/* HydratedSparseMatrix * &[Fr] (provekit/common/src/sparse_matrix.rs:606-624):
+ * CSR with row r spanning [row_starts[r], row_starts[r+1]) (the last row ends
+ * at nnz), u32 column indices, values expanded from the interner.  Fails like
+ * the reference's assert when z_len != num_cols. */
+int pkgpu_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
+                     const uint32_t* cols, const uint64_t* vals, const uint64_t* z, size_t z_len,
+                     uint64_t* out);
+int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                            size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                            const uint64_t* z, uint64_t* out, void* stream);
+/* calculate_witness_bounds (provekit/common/src/utils/sumcheck.rs:163-179):
+ * a = A w, b = B w, c = a o b, each zero-padded to padded_len =
+ * num_rows.next_power_of_two() (written to *padded_len; the out arrays must
+ * hold that many elements). */
+int pkgpu_calculate_witness_bounds(size_t num_rows, size_t num_cols, const uint32_t* a_row_starts,
+                                   size_t a_nnz, const uint32_t* a_cols, const uint64_t* a_vals,
+                                   const uint32_t* b_row_starts, size_t b_nnz,
+                                   const uint32_t* b_cols, const uint64_t* b_vals,
+                                   const uint64_t* w, size_t w_len, uint64_t* a_out,
+                                   uint64_t* b_out, uint64_t* c_out, size_t* padded_len);
+/* calculate_evaluations_over_boolean_hypercube_for_eq(r, num_entries)
+ * (sumcheck.rs:108-120): eq(r, x) for x < num_entries, r[0] on the top bit. */
+int pkgpu_eq_evaluations(const uint64_t* r, size_t m, size_t num_entries, uint64_t* out);
+int pkgpu_eq_evaluations_device(const uint64_t* r, size_t m, size_t num_entries, uint64_t* out,
+                                void* stream);
+/* sumcheck_fold_map_reduce([a, b, c, eq], fold, map) with the R1CS map of
+ * run_zk_sumcheck_prover (provekit/prover/src/whir_r1cs.rs:551-559), on
+ * device-resident MLEs of `size` elements.  fold: host pointer to one Fr or
+ * NULL (round 0).  With a fold the four MLEs are folded in place and the
+ * caller keeps the first size/2 elements (the reference's truncate).
+ * out: host, 3 Fr = (h(0), h(-1), h(inf)). */
+int pkgpu_sumcheck_r1cs_round_device(uint64_t* a, uint64_t* b, uint64_t* c, uint64_t* eq,
+                                     size_t size, const uint64_t* fold, uint64_t* out,
+                                     void* stream);
+/* The whole sumcheck of run_zk_sumcheck_prover on the device, with this
+ * library's Skyscraper transcript (seeded) in place of spongefish.
+ * a, b, c: device, 2^m elements each, consumed.  blinding: host m x 4 Fr
+ * (reference ZK mode: sum_g absorbed, rho squeezed, claim rho * sum_g) or
+ * NULL (plain mode: rho = 0, claim = sum eq (a b - c) computed on device).
+ * Outputs (host): coeffs m x 4 Fr, alphas m Fr, r m Fr (the eq point),
+ * final_claim 1 Fr.  Returns PKGPU_ERR_INTERNAL if a round identity fails. */
+int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
+                                const uint64_t* blinding, uint64_t seed, uint64_t* coeffs,
+                                uint64_t* alphas, uint64_t* r, uint64_t* final_claim,
+                                void* stream);
+
 /* ------------------------------------------------------------ commit */
 /* WHIR-style commitment of one polynomial of n coefficients (n, blowup powers
  * of two): RS-encode to n*blowup elements (RSFr order; coset != 0: on 5*H),

@@ -2,13 +2,17 @@
  *
  * The Fiat-Shamir transcript is spongefish in the reference (git rev fcc277f,
  * not vendored, so its challenge bytes cannot be reproduced).  This project
- * defines its own transcript from the pinned Skyscraper compression, and this
- * file restates it independently of the product for the parity tests:
- *   init(seed):   s = compress([seed, 0, 0, 0], [TAG0, TAG1, 0, 0])
- *   absorb(x):    s = compress(s, x)          (x: 4 Montgomery limbs of Fr)
- *   squeeze():    s = compress(s, [0,0,0,0]); the challenge is the Fr whose
- *                 Montgomery limbs are s (s < p because compress reduces)
- * and the ZK sumcheck driver run_zk_sumcheck_prover
+ * defines its own transcript from the pinned Skyscraper compression C, shaped
+ * so that a device can run it with few sequential compressions; this file
+ * restates it independently of the product for the parity tests:
+ *   init(seed):     s = C([seed, 0, 0, 0], [TAG0, TAG1, 0, 0])
+ *   absorb1(x):     s = C(s, x)
+ *   absorb4(x0..3): s = C(s, C(C(x0, x1), C(x2, x3)))
+ *   squeeze(k):     out_j = C(s, [j + 1, 0, 0, 0]) for j < k, then
+ *                   s = C(s, [0, 0, 0, 0]); a challenge is the Fr whose
+ *                   Montgomery limbs are out_j (C reduces, so out_j < p)
+ * Field elements are absorbed as their 4 Montgomery limbs.
+ * The ZK sumcheck driver restates run_zk_sumcheck_prover
  * (provekit/prover/src/whir_r1cs.rs:519-621) with
  * compute_blinding_coefficients_for_round (:427-492) and
  * sum_over_hypercube (:494-501).
@@ -28,13 +32,21 @@ static void tr_init(or_transcript* t, uint64_t seed) {
   uint64_t l[4] = {seed, 0, 0, 0}, r[4] = {TAG0, TAG1, 0, 0};
   or_sky_compress(l, r, t->s);
 }
-static void tr_absorb(or_transcript* t, ofr x) { or_sky_compress(t->s, x.l, t->s); }
-static ofr tr_squeeze(or_transcript* t) {
+static void tr_absorb1(or_transcript* t, ofr x) { or_sky_compress(t->s, x.l, t->s); }
+static void tr_absorb4(or_transcript* t, const ofr x[4]) {
+  uint64_t h1[4], h2[4], h[4];
+  or_sky_compress(x[0].l, x[1].l, h1);
+  or_sky_compress(x[2].l, x[3].l, h2);
+  or_sky_compress(h1, h2, h);
+  or_sky_compress(t->s, h, t->s);
+}
+static void tr_squeeze(or_transcript* t, size_t k, ofr* out) {
+  for (size_t j = 0; j < k; j++) {
+    uint64_t c[4] = {j + 1, 0, 0, 0};
+    or_sky_compress(t->s, c, out[j].l);
+  }
   static const uint64_t z[4] = {0, 0, 0, 0};
   or_sky_compress(t->s, z, t->s);
-  ofr c;
-  memcpy(c.l, t->s, 32);
-  return c;
 }
 
 /* compute_blinding_coefficients_for_round (whir_r1cs.rs:427-492) */
@@ -89,21 +101,21 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
   if (size < 2) size = 2;
   or_transcript tr;
   tr_init(&tr, seed);
-  for (size_t i = 0; i < m; i++) r_out[i] = tr_squeeze(&tr);
+  if (m) tr_squeeze(&tr, m, r_out);
   ofr* eq = (ofr*)calloc(size, sizeof(ofr));
   or_eq_evals(r_out, m, (size_t)1 << m, eq);
   ofr zero = {{0}};
   ofr rho = zero, S;
   if (blinding) {
     ofr sg = sum_over_hypercube(blinding, m);
-    tr_absorb(&tr, sg);
-    rho = tr_squeeze(&tr);
+    tr_absorb1(&tr, sg);
+    tr_squeeze(&tr, 1, &rho);
     S = ofr_mul(rho, sg);
   } else {
     S = zero;
     for (size_t i = 0; i < size; i++)
       S = ofr_add(S, ofr_mul(eq[i], ofr_sub(ofr_mul(a[i], b[i]), c[i])));
-    tr_absorb(&tr, S);
+    tr_absorb1(&tr, S);
   }
   ofr half = ofr_inv(ofr_add(OFR_ONE, OFR_ONE));
   int ok = 0;
@@ -125,8 +137,8 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
     co[1] = ofr_sub(ofr_sub(ofr_sub(ofr_sub(S, co[0]), co[0]), co[3]), co[2]);
     ofr chk = ofr_add(ofr_add(ofr_add(ofr_add(co[0], co[0]), co[1]), co[2]), co[3]);
     if (!ofr_eq(chk, S)) ok = -1;
-    for (int k = 0; k < 4; k++) tr_absorb(&tr, co[k]);
-    alphas[idx] = tr_squeeze(&tr);
+    tr_absorb4(&tr, co);
+    tr_squeeze(&tr, 1, &alphas[idx]);
     S = or_eval_cubic(co, alphas[idx]);
   }
   *final_claim = S;
@@ -134,11 +146,12 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
   return ok;
 }
 
-/* transcript primitives exported for tests */
+/* transcript primitives exported for tests: absorb1 each element, then
+ * squeeze n_sq challenges at once */
 void or_transcript_run(uint64_t seed, const ofr* absorbs, size_t n_abs, ofr* squeezes,
                        size_t n_sq) {
   or_transcript t;
   tr_init(&t, seed);
-  for (size_t i = 0; i < n_abs; i++) tr_absorb(&t, absorbs[i]);
-  for (size_t i = 0; i < n_sq; i++) squeezes[i] = tr_squeeze(&t);
+  for (size_t i = 0; i < n_abs; i++) tr_absorb1(&t, absorbs[i]);
+  if (n_sq) tr_squeeze(&t, n_sq, squeezes);
 }

@@ -74,6 +74,14 @@ SIGNATURES = {
     "pkgpu_poseidon2_merkle_graph": [_vp, _sz, _vp, ctypes.POINTER(_vp)],
     "pkgpu_graph_launch": [_vp, _vp],
     "pkgpu_graph_destroy": [_vp],
+    "pkgpu_sparse_mul": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp],
+    "pkgpu_sparse_mul_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_calculate_witness_bounds": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz,
+                                       _vp, _vp, _vp, ctypes.POINTER(_sz)],
+    "pkgpu_eq_evaluations": [_vp, _sz, _sz, _vp],
+    "pkgpu_eq_evaluations_device": [_vp, _sz, _sz, _vp, _vp],
+    "pkgpu_sumcheck_r1cs_round_device": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp],
+    "pkgpu_sumcheck_prove_device": [_vp, _vp, _vp, _sz, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
     "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
 }
@@ -311,6 +319,91 @@ def poseidon2_merkle_graph(leaves: "DeviceBuffer", n: int, nodes: "DeviceBuffer"
     h = ctypes.c_void_p()
     _call("pkgpu_poseidon2_merkle_graph", leaves.ptr, n, nodes.ptr, ctypes.byref(h))
     return Graph(h.value)
+
+
+# ----------------------------------------------------------------- R1CS / sumcheck
+class SparseMatrix:
+    """CSR as provekit's SparseMatrix (provekit/common/src/sparse_matrix.rs:56-71)
+    with interned values expanded: row_starts (u32, one per row), cols (u32),
+    vals ((nnz, 4) Montgomery limbs)."""
+
+    def __init__(self, num_rows, num_cols, row_starts, cols, vals):
+        self.num_rows, self.num_cols = int(num_rows), int(num_cols)
+        self.row_starts = np.ascontiguousarray(row_starts, dtype=np.uint32)
+        self.cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        self.vals = _fr_array(vals) if len(vals) else np.zeros((0, 4), np.uint64)
+
+    @property
+    def nnz(self):
+        return self.cols.shape[0]
+
+    def _args(self):
+        return (_ptr(self.row_starts) if self.num_rows else None, self.nnz,
+                _ptr(self.cols) if self.nnz else None, _ptr(self.vals) if self.nnz else None)
+
+    def __mul__(self, z):
+        """HydratedSparseMatrix * &[Fr] (sparse_matrix.rs:606-624)."""
+        _ensure_init()
+        zz = _fr_array(z) if len(z) else np.zeros((0, 4), np.uint64)
+        out = np.zeros((self.num_rows, 4), dtype=np.uint64)
+        rs, nnz, cols, vals = self._args()
+        _call("pkgpu_sparse_mul", self.num_rows, self.num_cols, rs, nnz, cols, vals,
+              _ptr(zz) if zz.size else None, zz.shape[0], _ptr(out) if out.size else None)
+        return out
+
+
+def calculate_witness_bounds(A: SparseMatrix, B: SparseMatrix, witness):
+    """calculate_witness_bounds (provekit/common/src/utils/sumcheck.rs:163-179)."""
+    _ensure_init()
+    w = _fr_array(witness)
+    n = 1
+    while n < A.num_rows:
+        n <<= 1
+    outs = [np.zeros((n, 4), np.uint64) for _ in range(3)]
+    plen = _sz()
+    ra, na, ca, va = A._args()
+    rb, nb, cb, vb = B._args()
+    _call("pkgpu_calculate_witness_bounds", A.num_rows, A.num_cols, ra, na, ca, va, rb, nb, cb, vb,
+          _ptr(w), w.shape[0], _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]), ctypes.byref(plen))
+    return tuple(o[: plen.value] for o in outs)
+
+
+def eq_evaluations(r, num_entries: int) -> np.ndarray:
+    """calculate_evaluations_over_boolean_hypercube_for_eq (sumcheck.rs:108-120)."""
+    _ensure_init()
+    rr = _fr_array(r) if len(r) else np.zeros((0, 4), np.uint64)
+    out = np.zeros((num_entries, 4), dtype=np.uint64)
+    _call("pkgpu_eq_evaluations", _ptr(rr) if rr.size else None, rr.shape[0], num_entries,
+          _ptr(out) if out.size else None)
+    return out
+
+
+def sumcheck_r1cs_round(a: "DeviceBuffer", b: "DeviceBuffer", c: "DeviceBuffer",
+                        eq: "DeviceBuffer", size: int, fold=None, stream=None) -> np.ndarray:
+    """One sumcheck_fold_map_reduce round on device-resident MLEs -> (3, 4)."""
+    out = np.zeros((3, 4), dtype=np.uint64)
+    f = np.ascontiguousarray(fold, dtype=np.uint64).reshape(4) if fold is not None else None
+    _call("pkgpu_sumcheck_r1cs_round_device", a.ptr, b.ptr, c.ptr, eq.ptr, size,
+          _ptr(f) if f is not None else None, _ptr(out), stream)
+    return out
+
+
+def sumcheck_prove(a, b, c, m: int, blinding=None, seed: int = 0, stream=None) -> dict:
+    """Whole sumcheck on the device (pkgpu_sumcheck_prove_device).  a, b, c are
+    host (2^m, 4) arrays (copied to the device) or DeviceBuffers (consumed)."""
+    _ensure_init()
+    bufs = []
+    for x in (a, b, c):
+        bufs.append(x if isinstance(x, DeviceBuffer) else DeviceBuffer.from_array(_fr_array(x)))
+    coeffs = np.zeros((max(m, 1), 4, 4), np.uint64)
+    alphas = np.zeros((max(m, 1), 4), np.uint64)
+    r = np.zeros((max(m, 1), 4), np.uint64)
+    fin = np.zeros(4, np.uint64)
+    g = np.ascontiguousarray(blinding, dtype=np.uint64).reshape(-1, 4) if blinding is not None else None
+    _call("pkgpu_sumcheck_prove_device", bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, m,
+          _ptr(g) if g is not None else None, seed, _ptr(coeffs), _ptr(alphas), _ptr(r), _ptr(fin),
+          stream)
+    return {"coeffs": coeffs[:m], "alphas": alphas[:m], "r": r[:m], "final": fin}
 
 
 # ----------------------------------------------------------------- commit

@@ -26,6 +26,7 @@ SOURCES = [
     "skyscraper.hip",
     "commit.hip",
     "poseidon2.hip",
+    "sumcheck.hip",
 ]
 
 ARCH = "gfx950"

@@ -7,6 +7,7 @@
 #include "host_fr.hpp"
 #include "kernels.hpp"
 #include "provekit.hpp"
+#include "sumcheck.hpp"
 
 using pk::Error;
 using pk::fr_t;
@@ -491,6 +492,200 @@ int pkgpu_graph_launch(void* graph, void* stream) {
 int pkgpu_graph_destroy(void* graph) {
   return guarded([&] {
     if (graph) PK_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+  });
+}
+
+namespace {
+pk::CsrDev csr_view(size_t rows, size_t cols_n, const uint32_t* rs, size_t nnz, const uint32_t* cols,
+                    const uint64_t* vals) {
+  pk::CsrDev A;
+  A.rows = rows;
+  A.ncols = cols_n;
+  A.nnz = nnz;
+  A.row_starts = rs;
+  A.cols = cols;
+  A.vals = reinterpret_cast<const fr_t*>(vals);
+  return A;
+}
+// Upload one host CSR matrix; the returned view points into `bufs`.
+pk::CsrDev upload_csr(size_t rows, size_t cols_n, const uint32_t* rs, size_t nnz,
+                      const uint32_t* cols, const uint64_t* vals, std::vector<pk::Scratch>& bufs,
+                      hipStream_t s) {
+  bufs.emplace_back(sizeof(uint32_t) * rows, s);
+  h2d(bufs.back().p, rs, sizeof(uint32_t) * rows, s);
+  const uint32_t* d_rs = bufs.back().as<uint32_t>();
+  bufs.emplace_back(sizeof(uint32_t) * nnz, s);
+  h2d(bufs.back().p, cols, sizeof(uint32_t) * nnz, s);
+  const uint32_t* d_cols = bufs.back().as<uint32_t>();
+  bufs.emplace_back(sizeof(fr_t) * nnz, s);
+  h2d(bufs.back().p, vals, sizeof(fr_t) * nnz, s);
+  return csr_view(rows, cols_n, d_rs, nnz, d_cols, bufs.back().as<uint64_t>());
+}
+void check_csr(size_t rows, const uint32_t* rs, size_t nnz, const uint32_t* cols,
+               const uint64_t* vals) {
+  REQ(rows == 0 || rs, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+  REQ(nnz == 0 || (cols && vals), pk::PK_ERR_ARG, "sparse matrix: null entries");
+}
+// Host-side CSR well-formedness: non-decreasing row starts within nnz, column
+// indices below num_cols (the device kernel trusts both).
+void validate_csr(size_t rows, size_t cols_n, const uint32_t* rs, size_t nnz,
+                  const uint32_t* cols) {
+  uint64_t prev = 0;
+  for (size_t r = 0; r < rows; r++) {
+    REQ(rs[r] >= prev && rs[r] <= nnz, pk::PK_ERR_ARG, "sparse matrix: bad row starts");
+    prev = rs[r];
+  }
+  for (size_t i = 0; i < nnz; i++)
+    REQ(cols[i] < cols_n, pk::PK_ERR_ARG, "sparse matrix: column index out of range");
+}
+}  // namespace
+
+int pkgpu_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
+                     const uint32_t* cols, const uint64_t* vals, const uint64_t* z, size_t z_len,
+                     uint64_t* out) {
+  return guarded([&] {
+    REQ(num_cols == z_len, pk::PK_ERR_SIZE, "Vector length does not match number of columns.");
+    check_csr(num_rows, row_starts, nnz, cols, vals);
+    REQ(num_rows == 0 || out, pk::PK_ERR_ARG, "sparse_mul: null output");
+    REQ(z_len == 0 || z, pk::PK_ERR_ARG, "sparse_mul: null vector");
+    validate_csr(num_rows, num_cols, row_starts, nnz, cols);
+    if (!num_rows) return;
+    hipStream_t s = pk::thread_stream();
+    std::vector<pk::Scratch> bufs;
+    pk::CsrDev A = upload_csr(num_rows, num_cols, row_starts, nnz, cols, vals, bufs, s);
+    Staged dz(sizeof(fr_t) * z_len, s), dout(sizeof(fr_t) * num_rows, s);
+    h2d(dz.get<void>(), z, sizeof(fr_t) * z_len, s);
+    pk::spmv_dev(A, dz.get<fr_t>(), dout.get<fr_t>(), s);
+    d2h(out, dout.get<void>(), sizeof(fr_t) * num_rows, s);
+    sync(s);
+  });
+}
+int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                            size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                            const uint64_t* z, uint64_t* out, void* stream) {
+  return guarded([&] {
+    check_csr(num_rows, row_starts, nnz, cols, vals);
+    REQ(num_rows == 0 || (out && (num_cols == 0 || z)), pk::PK_ERR_ARG, "sparse_mul: null pointer");
+    pk::spmv_dev(csr_view(num_rows, num_cols, row_starts, nnz, cols, vals),
+                 reinterpret_cast<const fr_t*>(z), reinterpret_cast<fr_t*>(out),
+                 stream_or_thread(stream));
+  });
+}
+int pkgpu_calculate_witness_bounds(size_t num_rows, size_t num_cols, const uint32_t* a_rs,
+                                   size_t a_nnz, const uint32_t* a_cols, const uint64_t* a_vals,
+                                   const uint32_t* b_rs, size_t b_nnz, const uint32_t* b_cols,
+                                   const uint64_t* b_vals, const uint64_t* w, size_t w_len,
+                                   uint64_t* a_out, uint64_t* b_out, uint64_t* c_out,
+                                   size_t* padded_len) {
+  return guarded([&] {
+    REQ(num_cols == w_len, pk::PK_ERR_SIZE, "Vector length does not match number of columns.");
+    check_csr(num_rows, a_rs, a_nnz, a_cols, a_vals);
+    check_csr(num_rows, b_rs, b_nnz, b_cols, b_vals);
+    REQ(padded_len && a_out && b_out && c_out && (w_len == 0 || w), pk::PK_ERR_ARG,
+        "calculate_witness_bounds: null pointer");
+    validate_csr(num_rows, num_cols, a_rs, a_nnz, a_cols);
+    validate_csr(num_rows, num_cols, b_rs, b_nnz, b_cols);
+    size_t n = 1;
+    while (n < num_rows) n <<= 1;
+    if (num_rows == 0) n = 0;
+    *padded_len = n;
+    if (!n) return;
+    hipStream_t s = pk::thread_stream();
+    std::vector<pk::Scratch> bufs;
+    pk::CsrDev A = upload_csr(num_rows, num_cols, a_rs, a_nnz, a_cols, a_vals, bufs, s);
+    pk::CsrDev B = upload_csr(num_rows, num_cols, b_rs, b_nnz, b_cols, b_vals, bufs, s);
+    Staged dw(sizeof(fr_t) * w_len, s), da(sizeof(fr_t) * n, s), db(sizeof(fr_t) * n, s),
+        dc(sizeof(fr_t) * n, s);
+    h2d(dw.get<void>(), w, sizeof(fr_t) * w_len, s);
+    pk::zero_dev(da.get<fr_t>() + num_rows, n - num_rows, s);
+    pk::zero_dev(db.get<fr_t>() + num_rows, n - num_rows, s);
+    pk::spmv_dev(A, dw.get<fr_t>(), da.get<fr_t>(), s);
+    pk::spmv_dev(B, dw.get<fr_t>(), db.get<fr_t>(), s);
+    pk::hadamard_dev(da.get<fr_t>(), db.get<fr_t>(), dc.get<fr_t>(), num_rows, n, s);
+    d2h(a_out, da.get<void>(), sizeof(fr_t) * n, s);
+    d2h(b_out, db.get<void>(), sizeof(fr_t) * n, s);
+    d2h(c_out, dc.get<void>(), sizeof(fr_t) * n, s);
+    sync(s);
+  });
+}
+int pkgpu_eq_evaluations(const uint64_t* r, size_t m, size_t num_entries, uint64_t* out) {
+  return guarded([&] {
+    if (!num_entries) return;
+    REQ(m < 63 && num_entries <= (size_t(1) << m), pk::PK_ERR_SIZE,
+        "assertion failed: num_entries <= full_size");
+    REQ(out && (m == 0 || r), pk::PK_ERR_ARG, "eq: null pointer");
+    hipStream_t s = pk::thread_stream();
+    Staged dr(sizeof(fr_t) * (m ? m : 1), s), dout(sizeof(fr_t) * num_entries, s);
+    h2d(dr.get<void>(), r, sizeof(fr_t) * m, s);
+    pk::eq_table_dev(dr.get<fr_t>(), (int)m, num_entries, dout.get<fr_t>(), s);
+    d2h(out, dout.get<void>(), sizeof(fr_t) * num_entries, s);
+    sync(s);
+  });
+}
+int pkgpu_eq_evaluations_device(const uint64_t* r, size_t m, size_t num_entries, uint64_t* out,
+                                void* stream) {
+  return guarded([&] {
+    if (!num_entries) return;
+    REQ(m < 63 && num_entries <= (size_t(1) << m), pk::PK_ERR_SIZE,
+        "assertion failed: num_entries <= full_size");
+    REQ(out && (m == 0 || r), pk::PK_ERR_ARG, "eq: null pointer");
+    pk::eq_table_dev(reinterpret_cast<const fr_t*>(r), (int)m, num_entries,
+                     reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
+int pkgpu_sumcheck_r1cs_round_device(uint64_t* a, uint64_t* b, uint64_t* c, uint64_t* eq,
+                                     size_t size, const uint64_t* fold, uint64_t* out,
+                                     void* stream) {
+  return guarded([&] {
+    REQ(a && b && c && eq && out, pk::PK_ERR_ARG, "sumcheck round: null pointer");
+    REQ(pk::is_pow2(size), pk::PK_ERR_NOT_POW2, "assertion failed: size.is_power_of_two()");
+    REQ(size >= 2, pk::PK_ERR_SIZE, "assertion failed: size >= 2");
+    REQ(!fold || size >= 4, pk::PK_ERR_SIZE, "assertion failed: size >= 4");
+    hipStream_t s = stream_or_thread(stream);
+    Staged work(sizeof(fr_t) * (pk::sc_partials_elems() + 4), s);
+    fr_t* partials = work.get<fr_t>();
+    fr_t* out3 = partials + pk::sc_partials_elems();
+    fr_t* dfold = nullptr;
+    if (fold) {
+      dfold = out3 + 3;
+      h2d(dfold, fold, 32, s);
+    }
+    pk::sc_round_dev(reinterpret_cast<fr_t*>(a), reinterpret_cast<fr_t*>(b),
+                     reinterpret_cast<fr_t*>(c), reinterpret_cast<fr_t*>(eq), size, dfold,
+                     partials, out3, s);
+    d2h(out, out3, sizeof(fr_t) * 3, s);
+    sync(s);
+  });
+}
+int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
+                                const uint64_t* blinding, uint64_t seed, uint64_t* coeffs,
+                                uint64_t* alphas, uint64_t* r, uint64_t* final_claim,
+                                void* stream) {
+  return guarded([&] {
+    REQ(a && b && c && final_claim && (m == 0 || (coeffs && alphas && r)), pk::PK_ERR_ARG,
+        "sumcheck: null pointer");
+    REQ(m <= (size_t)pk::kMaxSumcheckVars, pk::PK_ERR_SIZE, "sumcheck: more than 28 variables");
+    hipStream_t s = stream_or_thread(stream);
+    const size_t size = m ? (size_t(1) << m) : 2;
+    pk::ScState hst;
+    pk::sc_state_init_host(hst, (int)m, blinding ? 1 : 0, reinterpret_cast<const fr_t*>(blinding));
+    Staged dst(sizeof(pk::ScState), s), deq(sizeof(fr_t) * size, s),
+        work(sizeof(fr_t) * (pk::sc_partials_elems() + 3), s);
+    h2d(dst.get<void>(), &hst, sizeof hst, s);
+    pk::sc_prove_dev(dst.get<pk::ScState>(), reinterpret_cast<fr_t*>(a), reinterpret_cast<fr_t*>(b),
+                     reinterpret_cast<fr_t*>(c), deq.get<fr_t>(), (int)m, blinding ? 1 : 0, seed,
+                     work.get<fr_t>(), s);
+    d2h(&hst, dst.get<void>(), sizeof hst, s);
+    sync(s);
+    if (m) {
+      memcpy(coeffs, hst.coeffs, sizeof(fr_t) * 4 * m);
+      memcpy(alphas, hst.alpha, sizeof(fr_t) * m);
+      memcpy(r, hst.r, sizeof(fr_t) * m);
+    }
+    fr_t fin = hst.S;
+    memcpy(final_claim, &fin, 32);
+    REQ(!hst.error, pk::PK_ERR_INTERNAL,
+        "assertion `left == right` failed: sumcheck round identity c0 + c0 + c1 + c2 + c3 = S");
   });
 }
 

@@ -1,0 +1,445 @@
+// R1CS products and the ZK sumcheck on gfx950.
+//
+//  * k_spmv: HydratedSparseMatrix * &[Fr] (provekit/common/src/sparse_matrix.rs:606-624),
+//    one lane per row (R1CS rows hold a handful of entries), values already
+//    expanded from the interner.
+//  * eq table (provekit/common/src/utils/sumcheck.rs:108-153): eq(r, x) with
+//    r[0] on the top index bit, as eq_hi[x >> k] * eq_lo[x & (2^k - 1)] from two
+//    small tables, one product per entry.
+//  * k_sc_round: sumcheck_fold_map_reduce with the R1CS map
+//    (sumcheck.rs:14-102, provekit/prover/src/whir_r1cs.rs:552-559): optional
+//    in-place fold of the four MLEs, the three map outputs, a workgroup sum in
+//    LDS, one partial per workgroup.
+//  * k_sc_finalize: sums the partials, forms the round polynomial
+//    (whir_r1cs.rs:567-609, with compute_blinding_coefficients_for_round
+//    :427-492), runs the transcript step and writes the next challenge, all on
+//    one wave, so a whole sumcheck runs without the host.
+// The transcript is the project's Skyscraper transcript (init / absorb1 /
+// absorb4 / squeeze with parallel compressions, defined in sumcheck.hpp).
+#include "host_fr.hpp"
+#include "kernels.hpp"
+#include "sky.hpp"
+#include "sumcheck.hpp"
+
+namespace pk {
+
+namespace {
+
+inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+__device__ __forceinline__ fr_t fr_zero() {
+  fr_t z;
+#pragma unroll
+  for (int i = 0; i < 8; i++) z.l[i] = 0;
+  return z;
+}
+
+// ------------------------------------------------------------------ SpMV
+__global__ __launch_bounds__(256) void k_spmv(uint64_t rows, const uint32_t* row_starts,
+                                              uint64_t nnz, const uint32_t* cols,
+                                              const fr_t* vals, const fr_t* z, fr_t* out) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const uint64_t b = row_starts[r];
+  const uint64_t e = (r + 1 < rows) ? row_starts[r + 1] : nnz;
+  fr_t acc = fr_zero();
+  for (uint64_t i = b; i < e; i++) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + cols[i])));
+  fr_canon(acc);
+  fr_store(out + r, acc);
+}
+
+// c[i] = a[i] * b[i] for i < n, zero up to n_pad
+__global__ __launch_bounds__(256) void k_hadamard(const fr_t* a, const fr_t* b, fr_t* c,
+                                                  uint64_t n, uint64_t n_pad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  fr_t v = fr_zero();
+  if (i < n) {
+    v = fr_mul(fr_load(a + i), fr_load(b + i));
+    fr_canon(v);
+  }
+  fr_store(c + i, v);
+}
+
+__global__ void k_zero(fr_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fr_store(p + i, fr_zero());
+}
+
+// ------------------------------------------------------------------ eq
+// out[x] = prod_j (bit_{k-1-j}(x) ? r[j] : 1 - r[j]) for x < n, k variables.
+__global__ __launch_bounds__(256) void k_eq_direct(const fr_t* r, int k, uint64_t n, fr_t* out,
+                                                   fr_t one) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  fr_t acc = one;
+  for (int j = 0; j < k; j++) {
+    const fr_t rj = fr_load(r + j);
+    const fr_t f = ((x >> (k - 1 - j)) & 1) ? rj : fr_sub(one, rj);
+    acc = fr_mul(acc, f);
+  }
+  fr_canon(acc);
+  fr_store(out + x, acc);
+}
+__global__ __launch_bounds__(256) void k_eq_combine(const fr_t* hi, const fr_t* lo, int lo_bits,
+                                                    uint64_t n, fr_t* out) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  fr_t v = fr_mul(fr_load(hi + (x >> lo_bits)), fr_load(lo + (x & ((1ull << lo_bits) - 1))));
+  fr_canon(v);
+  fr_store(out + x, v);
+}
+
+// ------------------------------------------------------------------ rounds
+constexpr int kRoundThreads = 256;
+constexpr unsigned kMaxRoundBlocks = 1024;
+
+// Workgroup sum of three values per lane; lane 0 writes them to dst.
+__device__ __forceinline__ void block_sum3(fr_t v0, fr_t v1, fr_t v2, fr_t* dst) {
+  __shared__ fr_t sh[3][kRoundThreads];
+  const int t = threadIdx.x;
+  sh[0][t] = v0;
+  sh[1][t] = v1;
+  sh[2][t] = v2;
+  __syncthreads();
+  for (int w = kRoundThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) sh[q][t] = fr_add(sh[q][t], sh[q][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      fr_t s = sh[q][0];
+      fr_canon(s);
+      fr_store(dst + q, s);
+    }
+  }
+}
+
+// One sumcheck_fold_map_reduce call.  fold == nullptr: map over the halves of
+// size; else fold the quarters in place with *fold first, map over (p0, p1).
+__global__ __launch_bounds__(kRoundThreads) void k_sc_round(fr_t* a, fr_t* b, fr_t* c, fr_t* eq,
+                                                            uint64_t size, const fr_t* fold,
+                                                            fr_t* partials) {
+  const uint64_t h = fold ? size / 4 : size / 2;
+  fr_t f;
+  if (fold) f = fr_load(fold);
+  fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero();
+  fr_t* mle[4] = {a, b, c, eq};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < h;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    fr_t lo[4], hi[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      fr_t x0 = fr_load(mle[j] + i), x1 = fr_load(mle[j] + i + h);
+      if (fold) {
+        const fr_t x2 = fr_load(mle[j] + i + 2 * h), x3 = fr_load(mle[j] + i + 3 * h);
+        x0 = fr_add(x0, fr_mul(f, fr_sub(x2, x0)));
+        x1 = fr_add(x1, fr_mul(f, fr_sub(x3, x1)));
+        fr_t w0 = x0, w1 = x1;
+        fr_canon(w0);
+        fr_canon(w1);
+        fr_store(mle[j] + i, w0);
+        fr_store(mle[j] + i + h, w1);
+      }
+      lo[j] = x0;
+      hi[j] = x1;
+    }
+    // [eq0 (a0 b0 - c0), (2eq0 - eq1)((2a0 - a1)(2b0 - b1) - (2c0 - c1)),
+    //  (eq1 - eq0)(a1 - a0)(b1 - b0)]
+    const fr_t f0 = fr_mul(lo[3], fr_sub(fr_mul(lo[0], lo[1]), lo[2]));
+    const fr_t ea = fr_sub(fr_add(lo[3], lo[3]), hi[3]);
+    const fr_t aa = fr_sub(fr_add(lo[0], lo[0]), hi[0]);
+    const fr_t bb = fr_sub(fr_add(lo[1], lo[1]), hi[1]);
+    const fr_t cc = fr_sub(fr_add(lo[2], lo[2]), hi[2]);
+    const fr_t fm1 = fr_mul(ea, fr_sub(fr_mul(aa, bb), cc));
+    const fr_t finf = fr_mul(fr_mul(fr_sub(hi[3], lo[3]), fr_sub(hi[0], lo[0])), fr_sub(hi[1], lo[1]));
+    s0 = fr_add(s0, f0);
+    s1 = fr_add(s1, fm1);
+    s2 = fr_add(s2, finf);
+  }
+  block_sum3(s0, s1, s2, partials + 3 * blockIdx.x);
+}
+
+// sum_x eq(x) (a(x) b(x) - c(x)) partials: the claimed sum of the plain mode.
+__global__ __launch_bounds__(kRoundThreads) void k_sc_claim(const fr_t* a, const fr_t* b,
+                                                            const fr_t* c, const fr_t* eq,
+                                                            uint64_t size, fr_t* partials) {
+  fr_t s0 = fr_zero();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const fr_t v = fr_mul(fr_load(eq + i), fr_sub(fr_mul(fr_load(a + i), fr_load(b + i)), fr_load(c + i)));
+    s0 = fr_add(s0, v);
+  }
+  block_sum3(s0, fr_zero(), fr_zero(), partials + 3 * blockIdx.x);
+}
+
+// Sum n partial triples into out[0..3) (one workgroup).
+__global__ __launch_bounds__(kRoundThreads) void k_sum_partials(const fr_t* partials, unsigned n,
+                                                                fr_t* out) {
+  fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero();
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    s0 = fr_add(s0, fr_load(partials + 3 * i));
+    s1 = fr_add(s1, fr_load(partials + 3 * i + 1));
+    s2 = fr_add(s2, fr_load(partials + 3 * i + 2));
+  }
+  block_sum3(s0, s1, s2, out);
+}
+
+// ------------------------------------------------------------------ driver kernels
+__device__ __forceinline__ fr_t cubic(const fr_t* p, const fr_t& x) {
+  fr_t v = p[3];
+  v = fr_add(p[2], fr_mul(x, v));
+  v = fr_add(p[1], fr_mul(x, v));
+  v = fr_add(p[0], fr_mul(x, v));
+  fr_canon(v);
+  return v;
+}
+
+__device__ __forceinline__ fr_t limb_word(uint32_t w0, uint32_t w2) {
+  fr_t z = fr_zero();
+  z.l[0] = w0;
+  z.l[2] = w2;
+  return z;
+}
+
+// compute_blinding_coefficients_for_round (whir_r1cs.rs:427-492) for
+// compute_for < n (one lane).
+__device__ void blinding_coeffs(const ScState* st, int idx, fr_t out[4]) {
+  const int n = st->m;
+  const fr_t zero = fr_zero();
+  fr_t prefix = zero, suffix = zero;
+  for (int i = 0; i < idx; i++) prefix = fr_add(prefix, cubic(st->g[i], st->alpha[i]));
+  for (int i = idx + 1; i < n; i++) {
+    // g_i(0) + g_i(1) = 2 c0 + c1 + c2 + c3
+    const fr_t* gi = st->g[i];
+    suffix = fr_add(suffix, fr_add(fr_add(fr_add(gi[0], gi[0]), gi[1]), fr_add(gi[2], gi[3])));
+  }
+  fr_t pm = st->one;
+  for (int i = 0; i < n - 1 - idx; i++) pm = fr_add(pm, pm);
+  const fr_t sm = fr_mul(pm, st->half);
+  const fr_t other = fr_add(fr_mul(pm, prefix), fr_mul(sm, suffix));
+  const fr_t* cur = st->g[idx];
+  out[0] = fr_add(fr_mul(pm, cur[0]), other);
+  out[1] = fr_mul(pm, cur[1]);
+  out[2] = fr_mul(pm, cur[2]);
+  out[3] = fr_mul(pm, cur[3]);
+}
+
+// Transcript init and the m eq-point challenges: one wave, lanes in parallel.
+__global__ __launch_bounds__(64) void k_sc_init(ScState* st, uint64_t seed) {
+  __shared__ fr_t s;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    fr_t l = fr_zero(), r = fr_zero();
+    l.l[0] = (uint32_t)seed;
+    l.l[1] = (uint32_t)(seed >> 32);
+    r.l[0] = kTrTag0 & 0xffffffffu;
+    r.l[1] = (uint32_t)(kTrTag0 >> 32);
+    r.l[2] = kTrTag1 & 0xffffffffu;
+    r.l[3] = (uint32_t)(kTrTag1 >> 32);
+    s = sky_compress(l, r);
+  }
+  __syncthreads();
+  const int m = st->m;
+  // squeeze(m): r_j = C(s, [j + 1]) for lanes j < m, lane m: s = C(s, 0)
+  if (t < m) st->r[t] = sky_compress(s, limb_word(t + 1, 0));
+  fr_t ns;
+  if (t == m) ns = sky_compress(s, fr_zero());
+  __syncthreads();
+  if (t == m) st->state = ns;
+}
+
+// After the claim reduction: mode P absorbs the computed claim; mode R
+// absorbs sum_g, squeezes rho and sets the claim to rho * sum_g.
+__global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_t* claim) {
+  __shared__ fr_t s;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    if (st->mode == 0) {
+      st->S = claim[0];
+      st->rho = fr_zero();
+      s = sky_compress(st->state, claim[0]);
+    } else {
+      // sum_over_hypercube(g) (whir_r1cs.rs:494-501): p(0) + p(1) of round 0's coefficients
+      fr_t c[4];
+      blinding_coeffs(st, 0, c);
+      fr_t sg = fr_add(fr_add(fr_add(c[0], c[0]), c[1]), fr_add(c[2], c[3]));
+      fr_canon(sg);
+      st->sum_g = sg;
+      s = sky_compress(st->state, sg);
+    }
+  }
+  __syncthreads();
+  if (st->mode == 1) {
+    fr_t v;
+    if (t < 2) v = sky_compress(s, t == 0 ? limb_word(1, 0) : fr_zero());
+    __syncthreads();
+    if (t == 0) {
+      st->rho = v;
+      fr_t S = fr_mul(v, st->sum_g);
+      fr_canon(S);
+      st->S = S;
+    }
+    if (t == 1) st->state = v;
+  } else if (t == 0) {
+    st->state = s;
+  }
+}
+
+// Round idx: h = (h(0), h(-1), h(inf)) from the reduction; form the cubic,
+// check c0 + c0 + c1 + c2 + c3 = S, absorb4, squeeze alpha_idx, S = cubic(alpha).
+__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* h3, int idx) {
+  __shared__ fr_t co[4];
+  __shared__ fr_t hh[2];
+  __shared__ fr_t s1;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    fr_t g[4] = {fr_zero(), fr_zero(), fr_zero(), fr_zero()};
+    if (st->mode == 1) blinding_coeffs(st, idx, g);
+    const fr_t S = st->S, rho = st->rho;
+    const fr_t h0 = fr_load(h3), hm1 = fr_load(h3 + 1), hinf = fr_load(h3 + 2);
+    fr_t c0 = fr_add(h0, fr_mul(rho, g[0]));
+    const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
+    const fr_t at_m1 = fr_add(hm1, fr_mul(rho, gm1));
+    const fr_t three_c0 = fr_add(fr_add(c0, c0), c0);
+    fr_t c2 = fr_mul(st->half, fr_sub(fr_add(S, at_m1), three_c0));
+    fr_t c3 = fr_add(hinf, fr_mul(rho, g[3]));
+    fr_t c1 = fr_sub(fr_sub(fr_sub(fr_sub(S, c0), c0), c3), c2);
+    fr_canon(c0);
+    fr_canon(c1);
+    fr_canon(c2);
+    fr_canon(c3);
+    fr_t chk = fr_add(fr_add(fr_add(fr_add(c0, c0), c1), c2), c3);
+    fr_canon(chk);
+    fr_t Sc = S;
+    fr_canon(Sc);
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 8; q++) ok &= chk.l[q] == Sc.l[q];
+    if (!ok) st->error = 1;
+    co[0] = c0;
+    co[1] = c1;
+    co[2] = c2;
+    co[3] = c3;
+    st->coeffs[idx][0] = c0;
+    st->coeffs[idx][1] = c1;
+    st->coeffs[idx][2] = c2;
+    st->coeffs[idx][3] = c3;
+  }
+  __syncthreads();
+  // absorb4: lanes 0, 1 hash the coefficient pairs
+  fr_t v;
+  if (t < 2) v = sky_compress(co[2 * t], co[2 * t + 1]);
+  if (t < 2) hh[t] = v;
+  __syncthreads();
+  if (t == 0) s1 = sky_compress(st->state, sky_compress(hh[0], hh[1]));
+  __syncthreads();
+  // squeeze(1): lane 0 the challenge, lane 1 the next state
+  if (t < 2) v = sky_compress(s1, t == 0 ? limb_word(1, 0) : fr_zero());
+  __syncthreads();
+  if (t == 0) {
+    st->alpha[idx] = v;
+    st->S = cubic(st->coeffs[idx], v);
+  }
+  if (t == 1) st->state = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {
+  if (!A.rows) return;
+  k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.row_starts, A.nnz, A.cols, A.vals, z, out);
+  PK_HIP(hipGetLastError());
+}
+
+void hadamard_dev(const fr_t* a, const fr_t* b, fr_t* c, uint64_t n, uint64_t n_pad, hipStream_t s) {
+  if (!n_pad) return;
+  k_hadamard<<<nblk(n_pad, 256), 256, 0, s>>>(a, b, c, n, n_pad);
+  PK_HIP(hipGetLastError());
+}
+
+void zero_dev(fr_t* p, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_zero<<<nblk(n, 256), 256, 0, s>>>(p, n);
+  PK_HIP(hipGetLastError());
+}
+
+void eq_table_dev(const fr_t* r, int m, uint64_t num_entries, fr_t* out, hipStream_t s) {
+  if (!num_entries) return;
+  fr_t one;
+  memcpy(&one, &hf::ONE, 32);
+  if (m <= 12) {
+    k_eq_direct<<<nblk(num_entries, 256), 256, 0, s>>>(r, m, num_entries, out, one);
+    PK_HIP(hipGetLastError());
+    return;
+  }
+  const int lo_bits = m / 2, hi_bits = m - lo_bits;
+  Scratch hi(sizeof(fr_t) << hi_bits, s), lo(sizeof(fr_t) << lo_bits, s);
+  k_eq_direct<<<nblk(1ull << hi_bits, 256), 256, 0, s>>>(r, hi_bits, 1ull << hi_bits, hi.as<fr_t>(), one);
+  k_eq_direct<<<nblk(1ull << lo_bits, 256), 256, 0, s>>>(r + hi_bits, lo_bits, 1ull << lo_bits,
+                                                         lo.as<fr_t>(), one);
+  k_eq_combine<<<nblk(num_entries, 256), 256, 0, s>>>(hi.as<fr_t>(), lo.as<fr_t>(), lo_bits,
+                                                      num_entries, out);
+  PK_HIP(hipGetLastError());
+}
+
+static unsigned round_blocks(uint64_t work) {
+  uint64_t b = (work + kRoundThreads - 1) / kRoundThreads;
+  if (b < 1) b = 1;
+  if (b > kMaxRoundBlocks) b = kMaxRoundBlocks;
+  return (unsigned)b;
+}
+
+void sc_round_dev(fr_t* a, fr_t* b, fr_t* c, fr_t* eq, uint64_t size, const fr_t* fold,
+                  fr_t* partials, fr_t* out3, hipStream_t s) {
+  const uint64_t h = fold ? size / 4 : size / 2;
+  const unsigned nb = round_blocks(h);
+  k_sc_round<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, fold, partials);
+  k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, out3);
+  PK_HIP(hipGetLastError());
+}
+
+size_t sc_partials_elems() { return 3 * (size_t)kMaxRoundBlocks; }
+
+void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int mode,
+                  uint64_t seed, fr_t* work, hipStream_t s) {
+  const uint64_t size = m >= 1 ? (1ull << m) : 2;
+  fr_t* partials = work;
+  fr_t* h3 = work + sc_partials_elems();
+  k_sc_init<<<1, 64, 0, s>>>(st, seed);
+  PK_HIP(hipGetLastError());
+  eq_table_dev(st->r, m, 1ull << m, eq, s);
+  if (m == 0) zero_dev(eq + 1, 1, s);
+  if (mode == 0) {
+    const unsigned nb = round_blocks(size);
+    k_sc_claim<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, partials);
+    k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, h3);
+  }
+  k_sc_claim_finalize<<<1, 64, 0, s>>>(st, h3);
+  PK_HIP(hipGetLastError());
+  uint64_t cur = size;
+  for (int idx = 0; idx < m; idx++) {
+    const fr_t* fold = idx ? &st->alpha[idx - 1] : nullptr;
+    sc_round_dev(a, b, c, eq, cur, fold, partials, h3, s);
+    if (fold) cur /= 2;
+    k_sc_finalize<<<1, 64, 0, s>>>(st, h3, idx);
+    PK_HIP(hipGetLastError());
+  }
+}
+
+void sc_state_init_host(ScState& h, int m, int mode, const fr_t* blinding) {
+  memset(&h, 0, sizeof h);
+  h.m = m;
+  h.mode = mode;
+  memcpy(&h.one, &hf::ONE, 32);
+  hfr half = hf::inv(hf::from_u64(2));
+  memcpy(&h.half, &half, 32);
+  if (blinding) memcpy(h.g, blinding, sizeof(fr_t) * 4 * m);
+}
+
+}  // namespace pk

@@ -1,0 +1,267 @@
+"""R1CS products, eq tables and the sumcheck: the oracle pinned by the
+reference's tests (provekit/common/src/utils/sumcheck.rs:260-435) and by
+big-integer closed forms / the verifier's final equation
+(provekit/verifier/src/whir_r1cs.rs:249-257); the GPU bit-exact against it."""
+import random
+
+import numpy as np
+import pytest
+
+from provekit_amd.inputs import random_fr
+from tests.util import P, mont, unmont
+
+
+def fe(v):
+    return mont(v % P)
+
+
+def ints(arr):
+    return [unmont(x) for x in np.asarray(arr).reshape(-1, 4)]
+
+
+def frs(vals):
+    return np.array([fe(v) for v in vals], dtype=np.uint64).reshape(-1, 4)
+
+
+def eq_py(r, x):
+    acc = 1
+    for ri, xi in zip(r, x):
+        acc = acc * (ri * xi + (1 - ri) * (1 - xi)) % P
+    return acc
+
+
+# ---------------------------------------------------------------- CPU: oracle
+def test_calculate_eq_kats(oracle, kats):
+    k = kats["calculate_eq"]
+    assert unmont(oracle.calculate_eq(frs(k["r"]), frs(k["alpha"]))) == k["out"]
+    r = frs([0, 1, 1, 0])
+    assert unmont(oracle.calculate_eq(r, frs([0, 1, 1, 0]))) == 1
+    assert unmont(oracle.calculate_eq(r, frs([1, 0, 0, 1]))) == 0
+
+
+def test_eq_table_kats(oracle, kats):
+    r = [2, 3, 4, 5]
+    table = ints(oracle.eq_evals(frs(r), 16))
+    for i, v in enumerate(table):
+        x = [(i >> (3 - j)) & 1 for j in range(4)]  # r[0] on the top bit
+        assert v == eq_py(r, x)
+    assert ints(oracle.eq_evals(frs(r), 10)) == table[:10]
+    assert ints(oracle.eq_evals(frs(kats["eval_eq_left_only"]["r"]), 1)) == [kats["eval_eq_left_only"]["out"] % P]
+    assert ints(oracle.eq_evals(frs(kats["eq_single_entry"]["r"]), 1)) == [kats["eq_single_entry"]["out"]]
+    assert ints(oracle.eq_evals(np.zeros((0, 4), np.uint64), 1)) == [1]
+
+
+def csr_from_rows(rows, ncols):
+    starts, cols, vals = [], [], []
+    for row in rows:
+        starts.append(len(cols))
+        for v, c in sorted(row, key=lambda t: t[1]):
+            cols.append(c)
+            vals.append(v)
+    return np.array(starts, np.uint32), np.array(cols, np.uint32), frs(vals) if vals else np.zeros((0, 4), np.uint64)
+
+
+def transpose_rows(rows, ncols):
+    out = [[] for _ in range(ncols)]
+    for r, row in enumerate(rows):
+        for v, c in row:
+            out[c].append((v, r))
+    return out
+
+
+def test_r1cs_3x4_transposed_by_eq_alpha(oracle, kats):
+    # multiply_transposed_by_eq_alpha (sumcheck.rs:417-435) on the 3x4 R1CS
+    k = kats["r1cs_3x4"]
+    eqa = oracle.eq_evals(frs(k["alpha"]), 3)
+    for name, key in (("A", "expected_a"), ("B", "expected_b"), ("C", "expected_c")):
+        t = transpose_rows(k[name], 4)
+        rs, cs, vs = csr_from_rows(t, 3)
+        got = ints(oracle.sparse_mul(4, 3, rs, cs, vs, eqa))
+        assert got == [v % P for v in k[key]], name
+
+
+def map_py(a, b, c, e):
+    (a0, a1), (b0, b1), (c0, c1), (e0, e1) = a, b, c, e
+    f0 = e0 * (a0 * b0 - c0)
+    fm1 = (2 * e0 - e1) * ((2 * a0 - a1) * (2 * b0 - b1) - (2 * c0 - c1))
+    finf = (e1 - e0) * (a1 - a0) * (b1 - b0)
+    return [f0 % P, fm1 % P, finf % P]
+
+
+def round_py(mles, fold):
+    size = len(mles[0])
+    if fold is not None:
+        q = size // 4
+        for m in mles:
+            for i in range(q):
+                m[i] = (m[i] + fold * (m[i + 2 * q] - m[i])) % P
+                m[i + q] = (m[i + q] + fold * (m[i + 3 * q] - m[i + q])) % P
+        h = q
+    else:
+        h = size // 2
+    tot = [0, 0, 0]
+    for i in range(h):
+        o = map_py(*[(m[i], m[i + h]) for m in mles])
+        tot = [(x + y) % P for x, y in zip(tot, o)]
+    return tot
+
+
+def test_round_brute_force(oracle):
+    rng = random.Random(7)
+    size = 16
+    mles = [[rng.randrange(P) for _ in range(size)] for _ in range(4)]
+    arrs = [frs(m).copy() for m in mles]
+    assert ints(oracle.sumcheck_round(*arrs)) == round_py([list(m) for m in mles], None)
+    f = rng.randrange(P)
+    py = [list(m) for m in mles]
+    got = ints(oracle.sumcheck_round(*arrs, fold=fe(f)))
+    assert got == round_py(py, f)
+    for arr, m in zip(arrs, py):  # folded halves written back
+        assert ints(arr[: size // 2]) == m[: size // 2]
+
+
+def mle_eval(vals, point):
+    v = list(vals)
+    for x in point:
+        h = len(v) // 2
+        v = [(v[i] + x * (v[i + h] - v[i])) % P for i in range(h)]
+    return v[0]
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 6])
+def test_plain_sumcheck_verifier_equation(oracle, m):
+    rng = random.Random(m)
+    n = 1 << m
+    a, b, c = ([rng.randrange(P) for _ in range(n)] for _ in range(3))
+    out = oracle.sumcheck_prove(frs(a), frs(b), frs(c), m, None, seed=5)
+    assert out["ok"]
+    r, alpha = ints(out["r"]), ints(out["alphas"])
+    # the claimed sum is sum eq(r,x)(a b - c); each round's cubic hits it
+    claim = sum(eq_py(r, [(i >> (m - 1 - j)) & 1 for j in range(m)]) * (a[i] * b[i] - c[i]) for i in range(n)) % P
+    co = [ints(x) for x in out["coeffs"]]
+    S = claim
+    for k in range(m):
+        assert (2 * co[k][0] + co[k][1] + co[k][2] + co[k][3]) % P == S
+        x = alpha[k]
+        S = (co[k][0] + x * (co[k][1] + x * (co[k][2] + x * co[k][3]))) % P
+    assert unmont(out["final"]) == S
+    # verifier's final check: f(alpha) = eq(r, alpha) (a(alpha) b(alpha) - c(alpha))
+    fa = eq_py(r, alpha) * (mle_eval(a, alpha) * mle_eval(b, alpha) - mle_eval(c, alpha)) % P
+    assert unmont(out["final"]) == fa
+
+
+def test_reference_mode_rounds_consistent(oracle):
+    m = 5
+    n = 1 << m
+    rng = random.Random(11)
+    a = [rng.randrange(P) for _ in range(n)]
+    b = [rng.randrange(P) for _ in range(n)]
+    c = [x * y % P for x, y in zip(a, b)]  # satisfied: c = a o b (calculate_witness_bounds)
+    g = [rng.randrange(P) for _ in range(4 * m)]
+    out = oracle.sumcheck_prove(frs(a), frs(b), frs(c), m, frs(g), seed=3)
+    assert out["ok"]
+
+
+# ---------------------------------------------------------------- GPU
+def random_csr(rng, rows, cols, seed):
+    starts, cidx = [], []
+    for _ in range(rows):
+        starts.append(len(cidx))
+        k = rng.integers(1, 5)
+        cidx.extend(sorted(rng.choice(cols, size=k, replace=False).tolist()))
+    vals = random_fr(len(cidx), seed=seed, stream="csr")
+    return np.array(starts, np.uint32), np.array(cidx, np.uint32), vals
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("log_rows", [0, 4, 12, 18])
+def test_gpu_sparse_mul(gpu, oracle, log_rows):
+    rng = np.random.default_rng(log_rows)
+    rows = cols = 1 << log_rows
+    cols = max(cols, 4)
+    rs, cs, vs = random_csr(rng, rows, cols, log_rows)
+    z = random_fr(cols, seed=1, stream="z")
+    A = gpu.SparseMatrix(rows, cols, rs, cs, vs)
+    assert np.array_equal(A * z, oracle.sparse_mul(rows, cols, rs, cs, vs, z))
+
+
+@pytest.mark.gpu
+def test_gpu_sparse_mul_errors_and_kats(gpu, kats):
+    import provekit_amd as pk
+    A = gpu.SparseMatrix(2, 3, [0, 1], [0, 2], frs([5, 6]))
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        A * frs([1, 2])
+    assert "Vector length does not match number of columns." in str(e.value)
+    k = kats["r1cs_3x4"]
+    eqa = gpu.eq_evaluations(frs(k["alpha"]), 3)
+    for name, key in (("A", "expected_a"), ("B", "expected_b"), ("C", "expected_c")):
+        rs, cs, vs = csr_from_rows(transpose_rows(k[name], 4), 3)
+        assert ints(gpu.SparseMatrix(4, 3, rs, cs, vs) * eqa) == [v % P for v in k[key]]
+
+
+@pytest.mark.gpu
+def test_gpu_witness_bounds(gpu, oracle):
+    rng = np.random.default_rng(3)
+    rows, cols = 1000, 1024
+    A = gpu.SparseMatrix(rows, cols, *random_csr(rng, rows, cols, 10))
+    B = gpu.SparseMatrix(rows, cols, *random_csr(rng, rows, cols, 11))
+    w = random_fr(cols, seed=2, stream="w")
+    a, b, c = gpu.calculate_witness_bounds(A, B, w)
+    assert a.shape == (1024, 4)
+    ea = oracle.sparse_mul(rows, cols, A.row_starts, A.cols, A.vals, w)
+    eb = oracle.sparse_mul(rows, cols, B.row_starts, B.cols, B.vals, w)
+    assert np.array_equal(a[:rows], ea) and not a[rows:].any()
+    assert np.array_equal(b[:rows], eb) and not b[rows:].any()
+    assert ints(c[:rows]) == [x * y % P for x, y in zip(ints(ea), ints(eb))] and not c[rows:].any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m", [0, 1, 2, 5, 12, 13, 20])
+def test_gpu_eq(gpu, oracle, m):
+    r = random_fr(m, seed=m, stream="eq") if m else np.zeros((0, 4), np.uint64)
+    n = 1 << m
+    assert np.array_equal(gpu.eq_evaluations(r, n), oracle.eq_evals(r, n))
+    if n > 3:
+        assert np.array_equal(gpu.eq_evaluations(r, n - 3), oracle.eq_evals(r, n - 3))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m", [1, 2, 6, 16])
+def test_gpu_round(gpu, oracle, m):
+    n = 1 << m
+    arrs = [random_fr(n, seed=10 * m + j, stream="round") for j in range(4)]
+    bufs = [gpu.DeviceBuffer.from_array(x) for x in arrs]
+    ref = [x.copy() for x in arrs]
+    got = gpu.sumcheck_r1cs_round(*bufs, n)
+    exp = oracle.sumcheck_round(*ref)
+    assert np.array_equal(got, exp)
+    if n >= 4:
+        f = random_fr(1, seed=m, stream="fold")[0]
+        got = gpu.sumcheck_r1cs_round(*bufs, n, fold=f)
+        exp = oracle.sumcheck_round(*ref, fold=f)
+        assert np.array_equal(got, exp)
+        for buf, x in zip(bufs, ref):
+            assert np.array_equal(buf.download(shape=(n, 4))[: n // 2], x[: n // 2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,mode", [(1, "plain"), (2, "plain"), (5, "plain"), (10, "plain"),
+                                    (16, "plain"), (5, "ref"), (12, "ref")])
+def test_gpu_sumcheck_prove(gpu, oracle, m, mode):
+    n = 1 << m
+    a = random_fr(n, seed=m, stream="sa")
+    b = random_fr(n, seed=m, stream="sb")
+    if mode == "plain":
+        c = random_fr(n, seed=m, stream="sc")
+        g = None
+    else:
+        c = np.array([fe(unmont(x) * unmont(y)) for x, y in zip(a, b)], dtype=np.uint64)
+        g = random_fr(4 * m, seed=m, stream="g")
+    print(f"[case] sumcheck m={m} mode={mode}", flush=True)
+    got = gpu.sumcheck_prove(a, b, c, m, blinding=g, seed=77)
+    exp = oracle.sumcheck_prove(a, b, c, m, g, seed=77)
+    assert exp["ok"]
+    assert np.array_equal(got["r"], exp["r"])
+    assert np.array_equal(got["coeffs"], exp["coeffs"])
+    assert np.array_equal(got["alphas"], exp["alphas"])
+    assert np.array_equal(got["final"], exp["final"])
