@@ -202,6 +202,28 @@ int pkgpu_commit(const uint64_t* coeffs, size_t n, size_t blowup, int coset, siz
 int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int coset,
                         size_t leaf_elems, uint64_t* codeword, void* nodes, void* stream);
 
+/* ------------------------------------------------------------ batched commit */
+/* Many polynomials of uneven lengths committed together (no counterpart
+ * call in the reference, which commits one witness per whir call): a plan
+ * binds device buffers once; each run enqueues the coset scaling, one NTT
+ * launch per pass round for all polynomials, and ONE persistent launch that
+ * hashes every leaf and builds every tree from a device work queue.
+ * Polynomial i: coeffs[i] (n[i] Fr, power of two), blowup[i] (power of two),
+ * coset[i] (0: H, else 5*H), codewords[i] (n[i]*blowup[i] Fr),
+ * nodes[i] (2*leaves-1 digests, leaves = n[i]*blowup[i]/leaf_elems).
+ * tile_log (9..11): NTT tile of 2^tile_log elements (0: the tuned default). */
+int pkgpu_batch_plan_create(size_t count, const uint64_t* const* coeffs, const size_t* n,
+                            const size_t* blowup, const int* coset, size_t leaf_elems,
+                            uint64_t* const* codewords, void* const* nodes, int tile_log,
+                            void** plan);
+int pkgpu_batch_run(void* plan, void* stream);
+/* The run recorded once into a hipGraph (cached in the plan) and launched. */
+int pkgpu_batch_graph_launch(void* plan, void* stream);
+/* Copy the count roots of the last run to host (32 bytes each) after
+ * synchronising `stream`; fails if the hashing queue reported a timeout. */
+int pkgpu_batch_roots(void* plan, uint8_t* roots, void* stream);
+int pkgpu_batch_plan_destroy(void* plan);
+
 #ifdef __cplusplus
 }
 #endif

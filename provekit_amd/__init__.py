@@ -82,6 +82,11 @@ SIGNATURES = {
     "pkgpu_eq_evaluations_device": [_vp, _sz, _sz, _vp, _vp],
     "pkgpu_sumcheck_r1cs_round_device": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp],
     "pkgpu_sumcheck_prove_device": [_vp, _vp, _vp, _sz, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_batch_plan_create": [_sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _i, ctypes.POINTER(_vp)],
+    "pkgpu_batch_run": [_vp, _vp],
+    "pkgpu_batch_graph_launch": [_vp, _vp],
+    "pkgpu_batch_roots": [_vp, _vp, _vp],
+    "pkgpu_batch_plan_destroy": [_vp],
     "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
     "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
 }
@@ -489,3 +494,49 @@ def hash_many_device(size: int, inp: DeviceBuffer, count: int, out: DeviceBuffer
 
 def merkle_device(leaves: DeviceBuffer, n: int, nodes: DeviceBuffer, stream=None) -> None:
     _call("pkgpu_skyscraper_merkle_device", leaves.ptr, n, nodes.ptr, stream)
+
+
+class BatchCommit:
+    """A batched commitment plan (pkgpu_batch_plan_create) over device buffers:
+    polys is a list of (coeffs DeviceBuffer, n, blowup, coset)."""
+
+    def __init__(self, polys, leaf_elems: int = 16, tile_log: int = 0):
+        _ensure_init()
+        self.count = len(polys)
+        self.leaf_elems = leaf_elems
+        self.polys = polys
+        self.codewords, self.nodes = [], []
+        for (c, n, blowup, coset) in polys:
+            N = n * blowup
+            self.codewords.append(DeviceBuffer(32 * N))
+            self.nodes.append(DeviceBuffer(32 * (2 * (N // leaf_elems) - 1)))
+        arr = lambda vals, t: (t * len(vals))(*vals)  # noqa: E731
+        h = ctypes.c_void_p()
+        _call("pkgpu_batch_plan_create", self.count, arr([p[0].ptr for p in polys], _vp),
+              arr([p[1] for p in polys], _sz), arr([p[2] for p in polys], _sz),
+              arr([int(bool(p[3])) for p in polys], _i), leaf_elems,
+              arr([b.ptr for b in self.codewords], _vp), arr([b.ptr for b in self.nodes], _vp),
+              tile_log, ctypes.byref(h))
+        self.handle = h.value
+
+    def run(self, stream=None) -> None:
+        _call("pkgpu_batch_run", self.handle, stream)
+
+    def graph_launch(self, stream=None) -> None:
+        _call("pkgpu_batch_graph_launch", self.handle, stream)
+
+    def roots(self, stream=None) -> list:
+        out = np.zeros(32 * self.count, dtype=np.uint8)
+        _call("pkgpu_batch_roots", self.handle, _ptr(out), stream)
+        return [out[32 * i:32 * i + 32].tobytes() for i in range(self.count)]
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().pkgpu_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -27,6 +27,7 @@ SOURCES = [
     "commit.hip",
     "poseidon2.hip",
     "sumcheck.hip",
+    "batch.hip",
 ]
 
 ARCH = "gfx950"

@@ -7,6 +7,7 @@
 #include "host_fr.hpp"
 #include "kernels.hpp"
 #include "provekit.hpp"
+#include "batch.hpp"
 #include "sumcheck.hpp"
 
 using pk::Error;
@@ -706,6 +707,55 @@ int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int cos
                    leaf_elems, reinterpret_cast<fr_t*>(codeword), static_cast<fr_t*>(nodes),
                    stream_or_thread(stream));
   });
+}
+
+int pkgpu_batch_plan_create(size_t count, const uint64_t* const* coeffs, const size_t* n,
+                            const size_t* blowup, const int* coset, size_t leaf_elems,
+                            uint64_t* const* codewords, void* const* nodes, int tile_log,
+                            void** plan) {
+  return guarded([&] {
+    REQ(plan && count && coeffs && n && blowup && codewords && nodes, pk::PK_ERR_ARG,
+        "batch: null pointer");
+    std::vector<pk::BatchPolySpec> specs(count);
+    for (size_t i = 0; i < count; i++) {
+      specs[i].coeffs = reinterpret_cast<const fr_t*>(coeffs[i]);
+      specs[i].n = n[i];
+      specs[i].blowup = blowup[i];
+      specs[i].coset = coset ? coset[i] : 0;
+      specs[i].leaf_elems = leaf_elems;
+      specs[i].codeword = reinterpret_cast<fr_t*>(codewords[i]);
+      specs[i].nodes = static_cast<fr_t*>(nodes[i]);
+    }
+    *plan = pk::batch_plan_create(pk::device_current(), specs, tile_log ? tile_log : 11);
+  });
+}
+int pkgpu_batch_run(void* plan, void* stream) {
+  return guarded([&] {
+    REQ(plan, pk::PK_ERR_STATE, "batch: null plan");
+    pk::batch_run(*static_cast<pk::BatchPlan*>(plan), stream_or_thread(stream));
+  });
+}
+int pkgpu_batch_graph_launch(void* plan, void* stream) {
+  return guarded([&] {
+    REQ(plan, pk::PK_ERR_STATE, "batch: null plan");
+    hipGraphExec_t g = pk::batch_graph(*static_cast<pk::BatchPlan*>(plan));
+    PK_HIP(hipGraphLaunch(g, stream_or_thread(stream)));
+  });
+}
+int pkgpu_batch_roots(void* plan, uint8_t* roots, void* stream) {
+  return guarded([&] {
+    REQ(plan && roots, pk::PK_ERR_ARG, "batch: null pointer");
+    auto* p = static_cast<pk::BatchPlan*>(plan);
+    hipStream_t s = stream_or_thread(stream);
+    uint32_t err = 0;
+    d2h(&err, p->err, sizeof err, s);
+    d2h(roots, p->roots, 32 * (size_t)p->count, s);
+    sync(s);
+    REQ(!err, pk::PK_ERR_INTERNAL, "batch: hashing queue wait timed out");
+  });
+}
+int pkgpu_batch_plan_destroy(void* plan) {
+  return guarded([&] { delete static_cast<pk::BatchPlan*>(plan); });
 }
 
 }  // extern "C"

@@ -1,0 +1,318 @@
+// Batched commitment of many polynomials of uneven lengths.
+//
+//  1. coset scaling of every polynomial's coefficients: one launch;
+//  2. the NTT passes of all polynomials: one launch per pass round, the
+//     workgroup -> (polynomial, tile) map in a device table (k_ntt_pass_batch);
+//  3. all leaf hashing and all Merkle trees: ONE persistent launch fed by a
+//     device work queue (k_hash_queue).
+//
+// The queue holds, in topological order, leaf tasks (64 leaves of one
+// polynomial), node tasks (up to 64 nodes of one tree level) and one tail
+// task per tree (the last levels from <= 64 nodes to the root inside one
+// wave).  A wave takes the next task index from a global counter, waits for
+// the (at most two) tasks that produce its inputs, runs it and publishes a
+// completion flag.  Tasks are handed out in topological order, so every task a
+// wave waits for was already taken by a running wave that waits only on
+// earlier tasks: the earliest unfinished task always progresses, whatever the
+// residency.  Each wait is bounded; on timeout the wave raises the error flag
+// and leaves, so every wave exits.
+//
+// The upper tree levels of all polynomials overlap with other polynomials'
+// leaf hashing instead of running as a latency-bound tail per polynomial.
+#include <algorithm>
+
+#include "batch.hpp"
+#include "host_fr.hpp"
+#include "kernels.hpp"
+#include "sky.hpp"
+
+namespace pk {
+
+namespace {
+
+constexpr int kQueueWave = 64;
+constexpr uint64_t kSpinLimit = 1ull << 24;  // x ~64-cycle sleeps: well above any real wait
+
+__global__ __launch_bounds__(256) void k_coset_scale_batch(const BatchPolyDev* polys,
+                                                           const uint2* map, const fr_t* coset) {
+  const uint2 m = map[blockIdx.x];
+  const BatchPolyDev& p = polys[m.x];
+  const uint64_t i = (uint64_t)m.y * 256 + threadIdx.x;
+  if (i >= p.n) return;
+  fr_t v = fr_mul(fr_load(p.coeffs + i), fr_load(coset + i));
+  fr_canon(v);
+  fr_store(p.chunk + i, v);
+}
+
+__global__ void k_fill_zero_u32(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__device__ __forceinline__ bool wait_done(const uint32_t* done, int32_t dep, uint32_t* err) {
+  if (dep < 0) return true;
+  for (uint64_t it = 0; it < kSpinLimit; it++) {
+    if (__hip_atomic_load(done + dep, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return true;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+// Offset of tree level l (level 0 = n0 leaves) in the level-by-level node
+// array: sum_{i<l} n0 >> i = 2 n0 - 2 (n0 >> l).
+__device__ __forceinline__ uint64_t level_off(uint64_t n0, uint32_t l) {
+  return 2 * n0 - 2 * (n0 >> l);
+}
+
+__global__ __launch_bounds__(kQueueWave) void k_hash_queue(const BatchPolyDev* polys,
+                                                           const HashTask* tasks, uint32_t ntasks,
+                                                           uint32_t* done, uint32_t* next,
+                                                           uint32_t* err) {
+  __shared__ uint32_t tid_sh;
+  __shared__ fr_t lvl[kQueueWave];
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t == 0) tid_sh = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t id = tid_sh;
+    __syncthreads();
+    if (id >= ntasks) return;
+    const HashTask tk = tasks[id];
+    const BatchPolyDev& p = polys[tk.poly];
+    __shared__ int ok_sh;
+    if (t == 0) ok_sh = wait_done(done, tk.dep0, err) && wait_done(done, tk.dep1, err);
+    __syncthreads();
+    if (!ok_sh) return;
+    // the dependency flags were read with acquire by lane 0; make the data
+    // they guard visible to every lane of this wave before reading it
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tk.kind == kTaskLeaf) {
+      const uint64_t leaf = (uint64_t)tk.j * kQueueWave + t;
+      if (leaf < p.leaves) {
+        const fr_t* m = p.codeword + leaf * p.leaf_elems;
+        fr_t acc = fr_load(m);
+        for (uint64_t k = 1; k < p.leaf_elems; k++) acc = sky_compress(acc, fr_load(m + k));
+        fr_store(p.nodes + leaf, acc);
+      }
+    } else if (tk.kind == kTaskNode) {
+      const uint64_t nl = p.leaves >> tk.level;
+      const uint64_t node = (uint64_t)tk.j * kQueueWave + t;
+      if (node < nl) {
+        const fr_t* in = p.nodes + level_off(p.leaves, tk.level - 1);
+        fr_store(p.nodes + level_off(p.leaves, tk.level) + node,
+                 sky_compress(fr_load(in + 2 * node), fr_load(in + 2 * node + 1)));
+      }
+    } else {  // tail: level tk.level (<= 64 nodes) up to the root
+      uint64_t n = p.leaves >> tk.level;
+      uint64_t off = level_off(p.leaves, tk.level);
+      if ((uint64_t)t < n) lvl[t] = fr_load(p.nodes + off + t);
+      __syncthreads();
+      off += n;
+      for (; n > 1; n >>= 1) {
+        const uint64_t h = n >> 1;
+        fr_t v;
+        if ((uint64_t)t < h) v = sky_compress(lvl[2 * t], lvl[2 * t + 1]);
+        __syncthreads();
+        if ((uint64_t)t < h) {
+          lvl[t] = v;
+          fr_store(p.nodes + off + t, v);
+        }
+        __syncthreads();
+        off += h;
+      }
+    }
+    // publish: every lane's stores complete before the flag
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(done + id, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_gather_roots(const BatchPolyDev* polys, uint32_t count, fr_t* roots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const BatchPolyDev& p = polys[i];
+  fr_store(roots + i, fr_load(p.nodes + 2 * p.leaves - 2));
+}
+
+template <class T>
+T* upload_vec(const std::vector<T>& v, std::vector<void*>& owned) {
+  void* p = nullptr;
+  const size_t bytes = sizeof(T) * (v.empty() ? 1 : v.size());
+  PK_HIP(hipMalloc(&p, bytes));
+  owned.push_back(p);
+  if (!v.empty()) PK_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+BatchPlan::~BatchPlan() {
+  if (graph) (void)hipGraphExecDestroy(graph);
+  for (void* p : owned) (void)hipFree(p);
+}
+
+BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs, int tile_log) {
+  PK_REQUIRE(!specs.empty(), PK_ERR_ARG, "batch: no polynomials");
+  PK_REQUIRE(tile_log >= 9 && tile_log <= 11, PK_ERR_ARG, "batch: tile log must be 9..11");
+  auto plan = std::make_unique<BatchPlan>();
+  plan->count = (uint32_t)specs.size();
+  std::vector<BatchPolyDev> polys(specs.size());
+  std::vector<uint2> coset_map;
+  uint64_t chunk_total = 0, max_n = 0;
+  for (size_t i = 0; i < specs.size(); i++) {
+    const BatchPolySpec& s = specs[i];
+    PK_REQUIRE(is_pow2(s.n) && is_pow2(s.blowup) && s.coeffs && s.codeword && s.nodes,
+               PK_ERR_NOT_POW2, "batch: lengths and blowups must be powers of two");
+    PK_REQUIRE(s.leaf_elems >= 2 && is_pow2(s.leaf_elems) && s.n * s.blowup >= s.leaf_elems,
+               PK_ERR_SIZE, "batch: leaf size must be a power of two >= 2 dividing the codeword");
+    PK_REQUIRE(s.n * s.blowup <= (1ull << 28), PK_ERR_SIZE, "batch: codeword above 2^28");
+    if (s.coset) chunk_total += s.n;
+    max_n = std::max<uint64_t>(max_n, s.n);
+  }
+  fr_t* chunks = nullptr;
+  if (chunk_total) {
+    PK_HIP(hipMalloc((void**)&chunks, sizeof(fr_t) * chunk_total));
+    plan->owned.push_back(chunks);
+  }
+  const fr_t* coset = nullptr;
+  if (chunk_total) coset = coset_table(d, log2_exact(max_n));
+  plan->coset = coset;
+  uint64_t chunk_off = 0;
+  // NTT plans per polynomial
+  std::vector<std::vector<NttPassDesc>> nplans(specs.size());
+  size_t max_rounds = 0;
+  for (size_t i = 0; i < specs.size(); i++) {
+    const BatchPolySpec& s = specs[i];
+    BatchPolyDev& p = polys[i];
+    p.coeffs = s.coeffs;
+    p.codeword = s.codeword;
+    p.nodes = s.nodes;
+    p.n = s.n;
+    p.N = s.n * s.blowup;
+    p.leaf_elems = s.leaf_elems;
+    p.leaves = p.N / s.leaf_elems;
+    p.chunk = s.coset ? chunks + chunk_off : nullptr;
+    if (s.coset) {
+      for (uint64_t b = 0; b * 256 < s.n; b++) coset_map.push_back(make_uint2((uint32_t)i, (uint32_t)b));
+      chunk_off += s.n;
+    }
+    const fr_t* src = s.coset ? p.chunk : s.coeffs;
+    nplans[i] = ntt_plan(d, s.codeword, p.N, p.N, s.blowup, src, s.n, tile_log);
+    PK_REQUIRE(!nplans[i].empty(), PK_ERR_SIZE, "batch: blowup must be below the codeword size");
+    max_rounds = std::max(max_rounds, nplans[i].size());
+  }
+  plan->polys = upload_vec(polys, plan->owned);
+  plan->coset_map = upload_vec(coset_map, plan->owned);
+  plan->coset_blocks = (uint32_t)coset_map.size();
+  // pass rounds, grouped by tile size
+  std::vector<NttPassDesc> descs;
+  for (size_t r = 0; r < max_rounds; r++) {
+    for (int tl = 1; tl <= 11; tl++) {
+      std::vector<uint2> map;
+      for (size_t i = 0; i < specs.size(); i++) {
+        if (r >= nplans[i].size() || nplans[i][r].tl != tl) continue;
+        const uint32_t di = (uint32_t)descs.size();
+        descs.push_back(nplans[i][r]);
+        const uint64_t tiles = polys[i].N >> tl;
+        for (uint64_t t = 0; t < tiles; t++) map.push_back(make_uint2(di, (uint32_t)t));
+      }
+      if (map.empty()) continue;
+      BatchPlan::Round rd;
+      rd.tl = tl;
+      rd.blocks = (uint32_t)map.size();
+      rd.map = upload_vec(map, plan->owned);
+      plan->rounds.push_back(rd);
+    }
+  }
+  plan->descs = upload_vec(descs, plan->owned);
+  // hashing queue, topological: all leaf tasks, node tasks level by level, tails
+  std::vector<HashTask> tasks;
+  std::vector<std::vector<int32_t>> prev(specs.size());  // task ids of the previous level
+  int max_levels = 0;
+  std::vector<int> tail_level(specs.size());
+  for (size_t i = 0; i < specs.size(); i++) {
+    const uint64_t n0 = polys[i].leaves;
+    const int lg = log2_exact(n0);
+    tail_level[i] = lg > 6 ? lg - 6 : 0;  // first level with <= 64 nodes
+    max_levels = std::max(max_levels, tail_level[i]);
+    for (uint64_t j = 0; j * kQueueWave < n0; j++) {
+      prev[i].push_back((int32_t)tasks.size());
+      tasks.push_back(HashTask{(uint32_t)i, 0, (uint32_t)j, kTaskLeaf, -1, -1});
+    }
+  }
+  for (int l = 1; l <= max_levels; l++) {
+    for (size_t i = 0; i < specs.size(); i++) {
+      if (l > tail_level[i]) continue;
+      const uint64_t nl = polys[i].leaves >> l;
+      std::vector<int32_t> cur;
+      for (uint64_t j = 0; j * kQueueWave < nl; j++) {
+        const int32_t d0 = prev[i][2 * j];
+        const int32_t d1 = 2 * j + 1 < prev[i].size() ? prev[i][2 * j + 1] : -1;
+        cur.push_back((int32_t)tasks.size());
+        tasks.push_back(HashTask{(uint32_t)i, (uint32_t)l, (uint32_t)j, kTaskNode, d0, d1});
+      }
+      prev[i] = cur;
+    }
+  }
+  for (size_t i = 0; i < specs.size(); i++) {
+    if (polys[i].leaves <= 1) continue;
+    // level tail_level[i] was produced by exactly one task (<= 64 nodes)
+    tasks.push_back(HashTask{(uint32_t)i, (uint32_t)tail_level[i], 0, kTaskTail, prev[i][0], -1});
+  }
+  plan->ntasks = (uint32_t)tasks.size();
+  plan->tasks = upload_vec(tasks, plan->owned);
+  std::vector<uint32_t> zeros(plan->ntasks + 2, 0);
+  plan->done = upload_vec(zeros, plan->owned);  // [0, ntasks) flags, then counter, error
+  plan->next = plan->done + plan->ntasks;
+  plan->err = plan->next + 1;
+  fr_t* roots;
+  PK_HIP(hipMalloc((void**)&roots, sizeof(fr_t) * specs.size()));
+  plan->owned.push_back(roots);
+  plan->roots = roots;
+  int dev;
+  PK_HIP(hipGetDevice(&dev));
+  int cus = 0;
+  PK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  plan->queue_blocks = (uint32_t)cus * 16;
+  return plan.release();
+}
+
+void batch_run(BatchPlan& plan, hipStream_t s) {
+  const uint64_t nflags = (uint64_t)plan.ntasks + 2;
+  k_fill_zero_u32<<<(unsigned)((nflags + 255) / 256), 256, 0, s>>>(plan.done, nflags);
+  if (plan.coset_blocks)
+    k_coset_scale_batch<<<plan.coset_blocks, 256, 0, s>>>(plan.polys, plan.coset_map, plan.coset);
+  PK_HIP(hipGetLastError());
+  for (const BatchPlan::Round& rd : plan.rounds) ntt_batch_round_dev(plan.descs, rd.map, rd.blocks, rd.tl, s);
+  k_hash_queue<<<plan.queue_blocks, kQueueWave, 0, s>>>(plan.polys, plan.tasks, plan.ntasks,
+                                                         plan.done, plan.next, plan.err);
+  k_gather_roots<<<(plan.count + 63) / 64, 64, 0, s>>>(plan.polys, plan.count, plan.roots);
+  PK_HIP(hipGetLastError());
+}
+
+hipGraphExec_t batch_graph(BatchPlan& plan) {
+  if (plan.graph) return plan.graph;
+  hipStream_t cs;
+  PK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  PK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  try {
+    batch_run(plan, cs);
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    (void)hipStreamEndCapture(cs, &junk);
+    if (junk) (void)hipGraphDestroy(junk);
+    (void)hipStreamDestroy(cs);
+    throw;
+  }
+  PK_HIP(hipStreamEndCapture(cs, &g));
+  PK_HIP(hipGraphInstantiate(&plan.graph, g, nullptr, nullptr, 0));
+  PK_HIP(hipGraphDestroy(g));
+  PK_HIP(hipStreamDestroy(cs));
+  return plan.graph;
+}
+
+}  // namespace pk

@@ -1,0 +1,66 @@
+// Batched commitment of many polynomials (batch.hip).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace pk {
+
+// One polynomial of a batch (device pointers).
+struct BatchPolySpec {
+  const fr_t* coeffs;
+  uint64_t n;
+  uint64_t blowup;
+  int coset;
+  uint64_t leaf_elems;
+  fr_t* codeword;  // n * blowup
+  fr_t* nodes;     // 2 * leaves - 1
+};
+
+struct BatchPolyDev {
+  const fr_t* coeffs;
+  fr_t* chunk;     // coset-scaled coefficients, or null
+  fr_t* codeword;
+  fr_t* nodes;
+  uint64_t n, N, leaves, leaf_elems;
+};
+
+enum : uint32_t { kTaskLeaf = 0, kTaskNode = 1, kTaskTail = 2 };
+struct HashTask {
+  uint32_t poly, level, j, kind;
+  int32_t dep0, dep1;  // task ids producing the inputs (-1: none)
+};
+
+struct BatchPlan {
+  struct Round {
+    int tl;
+    uint32_t blocks;
+    const uint2* map;
+  };
+  uint32_t count = 0;
+  BatchPolyDev* polys = nullptr;
+  const uint2* coset_map = nullptr;
+  uint32_t coset_blocks = 0;
+  const fr_t* coset = nullptr;
+  NttPassDesc* descs = nullptr;
+  std::vector<Round> rounds;
+  HashTask* tasks = nullptr;
+  uint32_t ntasks = 0;
+  uint32_t* done = nullptr;  // ntasks completion flags, then the queue counter, then the error flag
+  uint32_t* next = nullptr;
+  uint32_t* err = nullptr;
+  uint32_t queue_blocks = 0;
+  fr_t* roots = nullptr;     // count roots after a run
+  hipGraphExec_t graph = nullptr;
+  std::vector<void*> owned;
+  ~BatchPlan();
+};
+
+BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs, int tile_log);
+// Enqueue one batched commit of every polynomial of the plan.
+void batch_run(BatchPlan& plan, hipStream_t s);
+// The same launches recorded once into an executable graph (cached in the plan).
+hipGraphExec_t batch_graph(BatchPlan& plan);
+
+}  // namespace pk

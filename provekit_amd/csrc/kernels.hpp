@@ -6,6 +6,23 @@
 namespace pk {
 
 // ---- NTT (ntt.hip) ----
+// One NTT pass over a tile grid: stages [s0, s0 + r) of codeword index bits,
+// tiles of 2^tl = 2^(r + jb) elements (see k_ntt_pass).
+struct NttPassDesc {
+  fr_t* data;
+  const fr_t* src;  // first pass only: read element g from src[g % chunk]
+  const fr_t* rr;   // root table
+  uint64_t chunk;
+  uint64_t Q;       // low stride: 2^q * M elements below the window
+  int r, jb, canon, tl;
+};
+// The passes of ntt_nr over `data` with tiles of at most 2^tile_log elements.
+std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t codeword,
+                                  uint64_t num_groups, const fr_t* src, uint64_t chunk,
+                                  int tile_log);
+// Run one batched pass round: workgroup b runs tile map[b].y of descs[map[b].x].
+void ntt_batch_round_dev(const NttPassDesc* descs, const uint2* map, uint32_t blocks, int tl,
+                         hipStream_t s);
 // In-place DIT NTT, natural -> bit-reversed order, over `len` elements of
 // `len / codeword` interleaved polynomials, starting at stage log2(num_groups)
 // (ntt::ntt_nr).  If `src` is given, the first pass reads element g from

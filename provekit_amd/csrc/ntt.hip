@@ -75,14 +75,7 @@ void build_power_table(fr_t* out, const hfr& base_h, int log, bool bit_reversed,
   PK_HIP(hipFreeAsync(hi, s));
 }
 
-struct PassArgs {
-  fr_t* data;
-  const fr_t* src;
-  const fr_t* rr;
-  uint64_t chunk;
-  uint64_t Q;  // low stride: 2^q * M elements below the window (M = interleave width)
-  int r, jb, canon;
-};
+using PassArgs = NttPassDesc;
 
 template <int TL>
 struct PassShape {
@@ -97,16 +90,14 @@ struct PassShape {
 //   h * (2^r * Q) + w * Q + l,   l = i_lo * M + c in [0, Q).
 // A tile is one h, all 2^r windows, and 2^jb consecutive l (2^jb divides Q).
 template <int TL>
-__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
+__device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, fr_t* lds) {
   constexpr int T = PassShape<TL>::T;
   constexpr int BPT = PassShape<TL>::BPT;
   constexpr int N = 1 << TL;
-  __shared__ fr_t lds[N];
   const int tid = threadIdx.x;
   const int jb = a.jb;
   const uint64_t Q = a.Q;
   const uint64_t tiles_per_h = Q >> jb;
-  const uint64_t tile = blockIdx.x;
   const uint64_t h = tile / tiles_per_h;
   const uint64_t j0 = (tile % tiles_per_h) << jb;
   const uint64_t base = h * (Q << a.r) + j0;
@@ -154,6 +145,22 @@ __global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
 }
 
 template <int TL>
+__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
+  __shared__ fr_t lds[1 << TL];
+  ntt_pass_tile<TL>(a, blockIdx.x, lds);
+}
+
+// Batched pass: workgroup b runs tile map[b].y of descriptor map[b].x, so one
+// launch advances many polynomials' NTTs (all with tiles of 2^TL elements).
+template <int TL>
+__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass_batch(const PassArgs* descs,
+                                                                     const uint2* map) {
+  __shared__ fr_t lds[1 << TL];
+  const uint2 m = map[blockIdx.x];
+  ntt_pass_tile<TL>(descs[m.x], m.y, lds);
+}
+
+template <int TL>
 void launch_pass(const PassArgs& a, uint64_t len, hipStream_t s) {
   const uint64_t tiles = len >> TL;
   k_ntt_pass<TL><<<(unsigned)tiles, PassShape<TL>::T, 0, s>>>(a);
@@ -177,6 +184,30 @@ void launch_pass_tl(int tl, const PassArgs& a, uint64_t len, hipStream_t s) {
   PK_HIP(hipGetLastError());
 }
 
+template <int TL>
+void launch_batch(const PassArgs* descs, const uint2* map, uint32_t blocks, hipStream_t s) {
+  k_ntt_pass_batch<TL><<<blocks, PassShape<TL>::T, 0, s>>>(descs, map);
+}
+
+void launch_pass_batch_tl(int tl, const PassArgs* descs, const uint2* map, uint32_t blocks,
+                          hipStream_t s) {
+  switch (tl) {
+    case 1: launch_batch<1>(descs, map, blocks, s); break;
+    case 2: launch_batch<2>(descs, map, blocks, s); break;
+    case 3: launch_batch<3>(descs, map, blocks, s); break;
+    case 4: launch_batch<4>(descs, map, blocks, s); break;
+    case 5: launch_batch<5>(descs, map, blocks, s); break;
+    case 6: launch_batch<6>(descs, map, blocks, s); break;
+    case 7: launch_batch<7>(descs, map, blocks, s); break;
+    case 8: launch_batch<8>(descs, map, blocks, s); break;
+    case 9: launch_batch<9>(descs, map, blocks, s); break;
+    case 10: launch_batch<10>(descs, map, blocks, s); break;
+    case 11: launch_batch<11>(descs, map, blocks, s); break;
+    default: throw Error(PK_ERR_INTERNAL, "ntt batch: bad tile size");
+  }
+  PK_HIP(hipGetLastError());
+}
+
 // Copy with replication: data[g] = src[g % chunk].
 __global__ void k_replicate(fr_t* data, const fr_t* src, uint64_t chunk, uint64_t len) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,8 +225,8 @@ struct NttPass {
 // tile; earlier passes split the rest evenly, each with at least 2^kMinRun
 // consecutive elements per run where the layout allows.  jb is capped by the
 // power of two dividing the low stride 2^q * M.
-static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_twos) {
-  constexpr int kTileLog = 11, kMinRun = 2;
+static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_twos, int kTileLog) {
+  constexpr int kMinRun = 2;
   std::vector<NttPass> out;
   int rem = s_end - s_begin;
   if (rem <= 0) return out;
@@ -216,8 +247,9 @@ static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_two
   return out;
 }
 
-void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
-                hipStream_t s, const fr_t* src, uint64_t chunk) {
+std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t codeword,
+                                  uint64_t num_groups, const fr_t* src, uint64_t chunk,
+                                  int tile_log) {
   PK_REQUIRE(codeword >= 1 && is_pow2(codeword) && len % codeword == 0, PK_ERR_SIZE,
              "ntt: length must be a multiple of the power-of-two codeword size");
   PK_REQUIRE(is_pow2(num_groups) && num_groups <= codeword, PK_ERR_SIZE,
@@ -227,18 +259,12 @@ void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t
   while (((M >> m_twos) & 1) == 0) m_twos++;
   const int L = log2_exact(codeword);
   const int s_begin = log2_exact(num_groups);
-  const int s_end = L;
-  std::vector<NttPass> plan = plan_passes(L, s_begin, s_end, m_twos);
-  if (plan.empty()) {
-    if (src) {
-      k_replicate<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(data, src, chunk, len);
-      PK_HIP(hipGetLastError());
-    }
-    return;
-  }
-  const fr_t* rr = roots_table(d, s_end - 1);
+  std::vector<NttPass> plan = plan_passes(L, s_begin, L, m_twos, tile_log);
+  std::vector<NttPassDesc> out;
+  if (plan.empty()) return out;
+  const fr_t* rr = roots_table(d, L - 1);
   for (size_t i = 0; i < plan.size(); i++) {
-    PassArgs a;
+    NttPassDesc a;
     a.data = data;
     a.src = (i == 0) ? src : nullptr;
     a.chunk = chunk;
@@ -247,8 +273,28 @@ void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t
     a.r = plan[i].r;
     a.jb = plan[i].jb;
     a.canon = (i + 1 == plan.size());
-    launch_pass_tl(plan[i].r + plan[i].jb, a, len, s);
+    a.tl = plan[i].r + plan[i].jb;
+    out.push_back(a);
   }
+  return out;
+}
+
+void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
+                hipStream_t s, const fr_t* src, uint64_t chunk) {
+  std::vector<NttPassDesc> plan = ntt_plan(d, data, len, codeword, num_groups, src, chunk, 11);
+  if (plan.empty()) {
+    if (src) {
+      k_replicate<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(data, src, chunk, len);
+      PK_HIP(hipGetLastError());
+    }
+    return;
+  }
+  for (const NttPassDesc& a : plan) launch_pass_tl(a.tl, a, len, s);
+}
+
+void ntt_batch_round_dev(const NttPassDesc* descs, const uint2* map, uint32_t blocks, int tl,
+                         hipStream_t s) {
+  if (blocks) launch_pass_batch_tl(tl, descs, map, blocks, s);
 }
 
 namespace {

@@ -1,0 +1,72 @@
+"""Batched uneven commitment (one persistent hashing launch from a device work
+queue): every polynomial's codeword, tree and root equal the oracle's plain
+commit of that polynomial, on plain runs and on hipGraph replays."""
+import numpy as np
+import pytest
+
+from provekit_amd.inputs import random_fr
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(gpu, ks, blowups, cosets, seed):
+    polys, host = [], []
+    for i, (k, b, c) in enumerate(zip(ks, blowups, cosets)):
+        coeffs = random_fr(1 << k, seed=seed + i, stream="batch")
+        host.append((coeffs, b, c))
+        polys.append((gpu.DeviceBuffer.from_array(coeffs), 1 << k, b, c))
+    return polys, host
+
+
+def check(gpu, oracle, batch, host, roots):
+    for i, (coeffs, b, c) in enumerate(host):
+        eroot, ecw, enodes = oracle.commit(coeffs, b, c, batch.leaf_elems)
+        N = coeffs.shape[0] * b
+        cw = batch.codewords[i].download(shape=(N, 4))
+        nodes = batch.nodes[i].download(np.uint8).reshape(-1, 32)
+        assert np.array_equal(cw, ecw), f"codeword {i}"
+        assert np.array_equal(nodes, enodes), f"tree {i}"
+        assert roots[i] == eroot, f"root {i}"
+
+
+def test_batch_uneven_small(gpu, oracle):
+    ks = [4, 9, 10, 11, 12, 13, 14, 9, 16, 12, 10, 3]
+    blowups = [2, 4, 2, 4, 2, 4, 2, 4, 2, 4, 2, 4]
+    cosets = [True, False] * 6
+    polys, host = make_batch(gpu, ks, blowups, cosets, seed=100)
+    print(f"[case] batch of {len(ks)} polys, k={ks}", flush=True)
+    batch = gpu.BatchCommit(polys, leaf_elems=16)
+    batch.run()
+    check(gpu, oracle, batch, host, batch.roots())
+    # a second run on the same plan gives the same result
+    for b in batch.nodes:
+        b.upload(np.zeros(b.nbytes // 8, np.uint64))
+    batch.run()
+    check(gpu, oracle, batch, host, batch.roots())
+
+
+def test_batch_graph_replays(gpu, oracle):
+    ks = [12, 15, 11, 17, 13]
+    polys, host = make_batch(gpu, ks, [4, 2, 4, 2, 4], [True] * 5, seed=7)
+    batch = gpu.BatchCommit(polys, leaf_elems=16)
+    for rep in range(3):
+        gpu.synchronize()
+        for b in batch.nodes:
+            b.upload(np.zeros(b.nbytes // 8, np.uint64))
+        for c in batch.codewords:
+            c.upload(np.zeros(c.nbytes // 8, np.uint64))
+        batch.graph_launch()
+        roots = batch.roots()
+        check(gpu, oracle, batch, host, roots)
+        gpu.poseidon2_hash(np.zeros((1, 4), np.uint64))  # eager work between replays
+
+
+def test_batch_matches_plain_device_commits(gpu):
+    ks = [14, 18, 12, 20]
+    polys, host = make_batch(gpu, ks, [4, 2, 2, 4], [True, True, False, True], seed=3)
+    batch = gpu.BatchCommit(polys, leaf_elems=16)
+    batch.run()
+    roots = batch.roots()
+    for i, (coeffs, b, c) in enumerate(host):
+        root, _, _ = gpu.commit(coeffs, b, c, 16, want_codeword=False, want_nodes=False)
+        assert root == roots[i], i
