@@ -224,6 +224,56 @@ int pkgpu_batch_graph_launch(void* plan, void* stream);
 int pkgpu_batch_roots(void* plan, uint8_t* roots, void* stream);
 int pkgpu_batch_plan_destroy(void* plan);
 
+/* ------------------------------------------------------------ serving loop */
+/* A request loop for R1CS sumcheck + witness-commit jobs with continuous
+ * batching (no counterpart in the reference, which proves one witness per
+ * process).  Each job: a, b, c = A z, B z, C z (2^m rows, 2^m columns), the
+ * whole sumcheck over eq(r, x)(a b - c) in plain mode with the library
+ * transcript seeded by `seed`, and the commit of z (blowup, coset,
+ * leaf_elems as pkgpu_commit).  Jobs are admitted in arrival order when a
+ * stream slot is free and their HBM footprint fits the budget; admitted jobs
+ * overlap on the device.  Inputs may be host or device pointers and must
+ * stay valid until the job has finished.  The codeword and tree stay resident
+ * (counted against the budget) until pkgpu_server_release. */
+typedef struct {
+  size_t m;
+  const uint32_t* row_starts[3]; /* A, B, C */
+  const uint32_t* cols[3];
+  const uint64_t* vals[3];
+  size_t nnz[3];
+  const uint64_t* z;
+  uint64_t seed;
+  size_t blowup;
+  int coset;
+  size_t leaf_elems;
+} pkgpu_job;
+
+typedef struct {
+  uint64_t root[4];        /* Merkle root of the witness commit (32 bytes) */
+  uint64_t final_claim[4]; /* last round's cubic at the last challenge */
+  double latency_ms;       /* submit -> finished */
+  double queue_ms;         /* submit -> admitted */
+  double run_ms;           /* admitted -> finished */
+  int status;              /* pkgpu_status of the job */
+} pkgpu_job_result;
+
+int pkgpu_server_create(int device, size_t hbm_budget_bytes, int slots, void** server);
+/* PKGPU_ERR_BUDGET if the job alone exceeds the budget. */
+int pkgpu_server_submit(void* server, const pkgpu_job* job, uint64_t* job_id);
+/* Blocks until the job finished (timeout_ms <= 0: no limit).  coeffs (m x 4
+ * Fr), alphas (m Fr), r (m Fr) may be NULL. */
+int pkgpu_server_wait(void* server, uint64_t job_id, double timeout_ms, pkgpu_job_result* result,
+                      uint64_t* coeffs, uint64_t* alphas, uint64_t* r);
+int pkgpu_server_poll(void* server, uint64_t job_id, int* done);
+/* Openings from the resident tree: for each leaf index, the leaf's
+ * leaf_elems elements (Fr) and the sibling digest at every level
+ * (log2(leaves) x 32 bytes, leaf level first). */
+int pkgpu_server_open(void* server, uint64_t job_id, const uint64_t* leaf_indices, size_t n,
+                      uint64_t* leaf_values, uint8_t* paths);
+int pkgpu_server_release(void* server, uint64_t job_id);
+int pkgpu_server_stats(void* server, uint64_t* bytes_in_use, uint64_t* peak_in_flight);
+int pkgpu_server_destroy(void* server);
+
 #ifdef __cplusplus
 }
 #endif

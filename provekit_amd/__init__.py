@@ -87,6 +87,14 @@ SIGNATURES = {
     "pkgpu_batch_graph_launch": [_vp, _vp],
     "pkgpu_batch_roots": [_vp, _vp, _vp],
     "pkgpu_batch_plan_destroy": [_vp],
+    "pkgpu_server_create": [_i, _sz, _i, ctypes.POINTER(_vp)],
+    "pkgpu_server_submit": [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)],
+    "pkgpu_server_wait": [_vp, ctypes.c_uint64, ctypes.c_double, _vp, _vp, _vp, _vp],
+    "pkgpu_server_poll": [_vp, ctypes.c_uint64, ctypes.POINTER(_i)],
+    "pkgpu_server_open": [_vp, ctypes.c_uint64, _vp, _sz, _vp, _vp],
+    "pkgpu_server_release": [_vp, ctypes.c_uint64],
+    "pkgpu_server_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+    "pkgpu_server_destroy": [_vp],
     "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
     "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
 }
@@ -533,6 +541,106 @@ class BatchCommit:
     def destroy(self) -> None:
         if self.handle:
             lib().pkgpu_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PkJob(ctypes.Structure):
+    """pkgpu_job (include/provekit_gpu.h)."""
+    _fields_ = [("m", _sz), ("row_starts", _vp * 3), ("cols", _vp * 3), ("vals", _vp * 3),
+                ("nnz", _sz * 3), ("z", _vp), ("seed", ctypes.c_uint64), ("blowup", _sz),
+                ("coset", _i), ("leaf_elems", _sz)]
+
+
+class PkJobResult(ctypes.Structure):
+    _fields_ = [("root", ctypes.c_uint64 * 4), ("final_claim", ctypes.c_uint64 * 4),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double), ("status", _i)]
+
+
+class Server:
+    """The serving loop (pkgpu_server_*).  submit() takes (A, B, C) as
+    SparseMatrix-like triples of arrays (or device buffers) and z."""
+
+    def __init__(self, hbm_budget_bytes: int, slots: int = 8, device: int = 0):
+        _ensure_init()
+        h = ctypes.c_void_p()
+        _call("pkgpu_server_create", device, hbm_budget_bytes, slots, ctypes.byref(h))
+        self.handle = h.value
+        self._keep = {}
+
+    def submit(self, m: int, mats, z, seed: int, blowup: int = 4, coset: bool = True,
+               leaf_elems: int = 16) -> int:
+        """mats: three (row_starts, cols, vals) triples, each array either a
+        numpy array (host) or a DeviceBuffer; z likewise."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else _ptr(x)
+        j = PkJob()
+        j.m = m
+        keep = []
+        for k, (rs, cs, vs) in enumerate(mats):
+            if not isinstance(rs, DeviceBuffer):
+                rs = np.ascontiguousarray(rs, dtype=np.uint32)
+                cs = np.ascontiguousarray(cs, dtype=np.uint32)
+                vs = _fr_array(vs)
+            keep += [rs, cs, vs]
+            j.row_starts[k] = ptr(rs)
+            j.cols[k] = ptr(cs)
+            j.vals[k] = ptr(vs)
+            j.nnz[k] = (cs.nbytes // 4) if isinstance(cs, DeviceBuffer) else cs.shape[0]
+        if not isinstance(z, DeviceBuffer):
+            z = _fr_array(z)
+        keep.append(z)
+        j.z = ptr(z)
+        j.seed, j.blowup, j.coset, j.leaf_elems = seed, blowup, int(bool(coset)), leaf_elems
+        jid = ctypes.c_uint64()
+        _call("pkgpu_server_submit", self.handle, ctypes.byref(j), ctypes.byref(jid))
+        self._keep[jid.value] = (keep, m)
+        return jid.value
+
+    def wait(self, jid: int, timeout_ms: float = 0.0) -> dict:
+        m = self._keep[jid][1]
+        res = PkJobResult()
+        coeffs = np.zeros((max(m, 1), 4, 4), np.uint64)
+        alphas = np.zeros((max(m, 1), 4), np.uint64)
+        r = np.zeros((max(m, 1), 4), np.uint64)
+        _call("pkgpu_server_wait", self.handle, jid, timeout_ms, ctypes.byref(res), _ptr(coeffs),
+              _ptr(alphas), _ptr(r))
+        self._keep[jid] = (None, m)
+        return {"root": bytes(np.array(res.root, dtype=np.uint64).tobytes()),
+                "final": np.array(res.final_claim, dtype=np.uint64),
+                "coeffs": coeffs[:m], "alphas": alphas[:m], "r": r[:m],
+                "latency_ms": res.latency_ms, "queue_ms": res.queue_ms, "run_ms": res.run_ms}
+
+    def poll(self, jid: int) -> bool:
+        d = ctypes.c_int()
+        _call("pkgpu_server_poll", self.handle, jid, ctypes.byref(d))
+        return bool(d.value)
+
+    def open(self, jid: int, leaf_indices, leaf_elems: int, depth: int):
+        idx = np.ascontiguousarray(leaf_indices, dtype=np.uint64)
+        vals = np.zeros((len(idx), leaf_elems, 4), np.uint64)
+        paths = np.zeros((len(idx), max(depth, 1), 32), np.uint8)
+        _call("pkgpu_server_open", self.handle, jid, _ptr(idx), len(idx), _ptr(vals), _ptr(paths))
+        return vals, paths[:, :depth]
+
+    def release(self, jid: int) -> None:
+        _call("pkgpu_server_release", self.handle, jid)
+        self._keep.pop(jid, None)
+
+    def stats(self) -> dict:
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _call("pkgpu_server_stats", self.handle, ctypes.byref(a), ctypes.byref(b))
+        return {"bytes_in_use": a.value, "peak_in_flight": b.value}
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().pkgpu_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -28,6 +28,7 @@ SOURCES = [
     "poseidon2.hip",
     "sumcheck.hip",
     "batch.hip",
+    "serve.hip",
 ]
 
 ARCH = "gfx950"

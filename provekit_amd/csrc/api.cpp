@@ -8,6 +8,7 @@
 #include "kernels.hpp"
 #include "provekit.hpp"
 #include "batch.hpp"
+#include "serve.hpp"
 #include "sumcheck.hpp"
 
 using pk::Error;
@@ -756,6 +757,96 @@ int pkgpu_batch_roots(void* plan, uint8_t* roots, void* stream) {
 }
 int pkgpu_batch_plan_destroy(void* plan) {
   return guarded([&] { delete static_cast<pk::BatchPlan*>(plan); });
+}
+
+static bool is_host_pointer(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;  // unregistered pageable memory
+  }
+  return at.type != hipMemoryTypeDevice;
+}
+
+int pkgpu_server_create(int device, size_t hbm_budget_bytes, int slots, void** server) {
+  return guarded([&] {
+    REQ(server, pk::PK_ERR_ARG, "server: null out-pointer");
+    *server = new pk::Server(device, hbm_budget_bytes, slots);
+  });
+}
+int pkgpu_server_submit(void* server, const pkgpu_job* job, uint64_t* job_id) {
+  return guarded([&] {
+    REQ(server && job && job_id, pk::PK_ERR_ARG, "server: null pointer");
+    pk::JobSpec sp;
+    sp.m = (int)job->m;
+    const size_t n = job->m < 63 ? (size_t(1) << job->m) : 0;
+    for (int k = 0; k < 3; k++) {
+      sp.rs[k] = job->row_starts[k];
+      sp.cols[k] = job->cols[k];
+      sp.vals[k] = reinterpret_cast<const fr_t*>(job->vals[k]);
+      sp.nnz[k] = job->nnz[k];
+      // host-side structure check for host-resident matrices (device ones are
+      // trusted: the kernels index z with their column indices)
+      if (sp.rs[k] && is_host_pointer(sp.rs[k]) && (!sp.nnz[k] || is_host_pointer(sp.cols[k])))
+        validate_csr(n, n, sp.rs[k], sp.nnz[k], sp.cols[k]);
+    }
+    sp.z = reinterpret_cast<const fr_t*>(job->z);
+    sp.seed = job->seed;
+    sp.blowup = job->blowup;
+    sp.coset = job->coset;
+    sp.leaf_elems = job->leaf_elems;
+    *job_id = static_cast<pk::Server*>(server)->submit(sp);
+  });
+}
+int pkgpu_server_wait(void* server, uint64_t job_id, double timeout_ms, pkgpu_job_result* result,
+                      uint64_t* coeffs, uint64_t* alphas, uint64_t* r) {
+  return guarded([&] {
+    REQ(server && result, pk::PK_ERR_ARG, "server: null pointer");
+    pk::JobResult jr = static_cast<pk::Server*>(server)->wait(job_id, timeout_ms);
+    memcpy(result->root, &jr.root, 32);
+    memcpy(result->final_claim, &jr.final_claim, 32);
+    result->latency_ms = jr.latency_ms;
+    result->queue_ms = jr.queue_ms;
+    result->run_ms = jr.run_ms;
+    result->status = jr.status;
+    if (coeffs && !jr.coeffs.empty()) memcpy(coeffs, jr.coeffs.data(), 32 * jr.coeffs.size());
+    if (alphas && !jr.alphas.empty()) memcpy(alphas, jr.alphas.data(), 32 * jr.alphas.size());
+    if (r && !jr.r.empty()) memcpy(r, jr.r.data(), 32 * jr.r.size());
+    REQ(jr.status == 0, jr.status, "server: job failed: " + jr.error);
+  });
+}
+int pkgpu_server_poll(void* server, uint64_t job_id, int* done) {
+  return guarded([&] {
+    REQ(server && done, pk::PK_ERR_ARG, "server: null pointer");
+    *done = static_cast<pk::Server*>(server)->poll(job_id) ? 1 : 0;
+  });
+}
+int pkgpu_server_open(void* server, uint64_t job_id, const uint64_t* leaf_indices, size_t n,
+                      uint64_t* leaf_values, uint8_t* paths) {
+  return guarded([&] {
+    REQ(server && (n == 0 || (leaf_indices && leaf_values && paths)), pk::PK_ERR_ARG,
+        "server: null pointer");
+    static_cast<pk::Server*>(server)->open(job_id, leaf_indices, n,
+                                           reinterpret_cast<fr_t*>(leaf_values),
+                                           reinterpret_cast<fr_t*>(paths));
+  });
+}
+int pkgpu_server_release(void* server, uint64_t job_id) {
+  return guarded([&] {
+    REQ(server, pk::PK_ERR_ARG, "server: null pointer");
+    static_cast<pk::Server*>(server)->release(job_id);
+  });
+}
+int pkgpu_server_stats(void* server, uint64_t* bytes_in_use, uint64_t* peak_in_flight) {
+  return guarded([&] {
+    REQ(server, pk::PK_ERR_ARG, "server: null pointer");
+    auto* s = static_cast<pk::Server*>(server);
+    if (bytes_in_use) *bytes_in_use = s->bytes_in_use();
+    if (peak_in_flight) *peak_in_flight = s->peak_in_flight();
+  });
+}
+int pkgpu_server_destroy(void* server) {
+  return guarded([&] { delete static_cast<pk::Server*>(server); });
 }
 
 }  // extern "C"

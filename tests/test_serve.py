@@ -1,0 +1,122 @@
+"""Serving loop: each job (Az, Bz, Cz -> sumcheck -> witness commit) equals
+the oracle, and the same job gives the same result alone and in a concurrent
+mix; openings come from the resident tree; HBM-budget admission."""
+import numpy as np
+import pytest
+
+from provekit_amd.inputs import random_fr
+from tests.util import P
+
+pytestmark = pytest.mark.gpu
+
+
+def make_csr(rng, n, seed, stream):
+    k = rng.integers(1, 5, size=n)
+    cand = np.sort(rng.integers(0, n, size=(n, 4)), axis=1)
+    keep = np.ones((n, 4), bool)
+    keep[:, 1:] = cand[:, 1:] != cand[:, :-1]           # dedupe within a row
+    keep &= np.arange(4)[None, :] < k[:, None]           # 1..4 candidates per row
+    counts = keep.sum(axis=1)
+    rs = np.zeros(n, np.uint32)
+    rs[1:] = np.cumsum(counts)[:-1]
+    cols = cand[keep].astype(np.uint32)
+    vals = random_fr(cols.shape[0], seed=seed, stream=stream)
+    return rs, cols, vals
+
+
+def make_job(m, seed):
+    rng = np.random.default_rng(seed)
+    n = 1 << m
+    mats = [make_csr(rng, n, seed, f"job{k}") for k in range(3)]
+    z = random_fr(n, seed=seed, stream="z")
+    z[0] = np.array([0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E, 0x0E0A77C19A07DF2F],
+                    dtype=np.uint64)  # z[0] = 1 (Montgomery form of one)
+    return mats, z
+
+
+def expected(oracle, m, mats, z, seed):
+    n = 1 << m
+    abc = [oracle.sparse_mul(n, n, rs, cs, vs, z) for rs, cs, vs in mats]
+    sc = oracle.sumcheck_prove(abc[0], abc[1], abc[2], m, None, seed=seed)
+    root, cw, nodes = oracle.commit(z, 4, True, 16)
+    return sc, root, nodes
+
+
+def test_job_matches_oracle_and_openings(gpu, oracle):
+    srv = gpu.Server(hbm_budget_bytes=4 << 30, slots=2)
+    m = 10
+    mats, z = make_job(m, 5)
+    jid = srv.submit(m, mats, z, seed=99)
+    res = srv.wait(jid, timeout_ms=60000)
+    sc, root, nodes = expected(oracle, m, mats, z, 99)
+    assert sc["ok"]
+    assert res["root"] == root
+    assert np.array_equal(res["coeffs"], sc["coeffs"])
+    assert np.array_equal(res["alphas"], sc["alphas"])
+    assert np.array_equal(res["r"], sc["r"])
+    assert np.array_equal(res["final"], sc["final"])
+    # openings: leaf values and sibling path from the resident tree
+    leaves = (4 << m) // 16
+    depth = leaves.bit_length() - 1
+    idx = [0, 1, 37, leaves - 1]
+    vals, paths = srv.open(jid, idx, 16, depth)
+    _, cw, _ = oracle.commit(z, 4, True, 16)
+    for q, i in enumerate(idx):
+        assert np.array_equal(vals[q], cw[16 * i:16 * i + 16])
+        off, n_l = 0, leaves
+        for l in range(depth):
+            sib = (i >> l) ^ 1
+            assert paths[q, l].tobytes() == nodes[off + sib].tobytes()
+            off += n_l
+            n_l //= 2
+    srv.release(jid)
+    import provekit_amd as pk
+    with pytest.raises(pk.ProvekitGpuError):
+        srv.open(jid, [0], 16, depth)
+    srv.destroy()
+
+
+def test_mix_equals_alone(gpu):
+    jobs = [(m, make_job(m, 100 + i)) for i, m in enumerate([12, 10, 14, 11, 13, 10, 15, 12])]
+    alone = []
+    srv1 = gpu.Server(hbm_budget_bytes=8 << 30, slots=1)
+    for i, (m, (mats, z)) in enumerate(jobs):
+        jid = srv1.submit(m, mats, z, seed=1000 + i)
+        alone.append(srv1.wait(jid, timeout_ms=120000))
+        srv1.release(jid)
+    srv1.destroy()
+    srv = gpu.Server(hbm_budget_bytes=8 << 30, slots=6)
+    ids = [srv.submit(m, mats, z, seed=1000 + i) for i, (m, (mats, z)) in enumerate(jobs)]
+    mixed = [srv.wait(j, timeout_ms=120000) for j in ids]
+    st = srv.stats()
+    for j in ids:
+        srv.release(j)
+    srv.destroy()
+    print(f"[case] mix peak_in_flight={st['peak_in_flight']}", flush=True)
+    assert st["peak_in_flight"] >= 2  # the mix actually overlapped on the device
+    for a, b in zip(alone, mixed):
+        assert a["root"] == b["root"]
+        for key in ("coeffs", "alphas", "r", "final"):
+            assert np.array_equal(a[key], b[key])
+
+
+def test_budget_admission(gpu):
+    import provekit_amd as pk
+    m = 12
+    mats, z = make_job(m, 7)
+    tiny = gpu.Server(hbm_budget_bytes=1 << 20, slots=4)
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        tiny.submit(m, mats, z, seed=1)
+    assert e.value.code == pk.ERR_BUDGET
+    tiny.destroy()
+    # a budget that holds one job at a time serialises admission
+    srv = gpu.Server(hbm_budget_bytes=4 << 20, slots=4)  # one m=12 job is ~2.6 MB
+    ids = [srv.submit(m, mats, z, seed=s) for s in range(3)]
+    outs = []
+    for j in ids:
+        outs.append(srv.wait(j, timeout_ms=60000))
+        srv.release(j)  # frees the resident commit so the next job fits
+    assert srv.stats()["peak_in_flight"] == 1
+    assert outs[0]["root"] == outs[1]["root"] == outs[2]["root"]  # same witness
+    assert not np.array_equal(outs[0]["alphas"], outs[1]["alphas"])  # different seeds
+    srv.destroy()
