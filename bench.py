@@ -135,7 +135,7 @@ def cpu_baseline(coeffs, blowup, leaf_elems, budget_s):
 
 
 def run_batched(pk, hip, s, seed, steps, warmup, leaf_elems=16):
-    """Config 3: 64 polynomials, k ~ U{12..22}, blowup 2 or 4 by coin flip, coset
+    """Uneven batch: 64 polynomials, k ~ U{12..22}, blowup 2 or 4 by coin flip, coset
     NTT, one tree each; batched (one persistent hashing launch) vs 64 plain
     per-call commits on the same GPU, everything resident in HBM."""
     from provekit_amd.inputs import random_fr
@@ -210,6 +210,189 @@ def run_batched(pk, hip, s, seed, steps, warmup, leaf_elems=16):
     }
 
 
+def pool_fr(n, seed, stream):
+    """n Fr elements: seeded rotations of one ChaCha20 pool of at most 2^22
+    (the generator costs ~1 us per element on the host)."""
+    from provekit_amd.inputs import random_fr
+    m = min(n, 1 << 22)
+    pool = random_fr(m, seed=seed, stream=stream)
+    if n == m:
+        return pool
+    rng = np.random.default_rng(seed)
+    parts = [np.roll(pool, -int(rng.integers(0, m)), axis=0) for _ in range((n + m - 1) // m)]
+    return np.ascontiguousarray(np.concatenate(parts)[:n])
+
+
+def base_line(metric, value, unit, steps, warmup, ms_per_step, higher, dtype, config, extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+           "config": config}
+    out.update(extra)
+    return out
+
+
+def workload_ntt(pk, hip, s, args):
+    """Plain forward NTT of 2^16 uniform Fr (natural -> bit-reversed)."""
+    from oracle import pyoracle
+    from provekit_amd.inputs import random_fr
+
+    n = 1 << 16
+    v = random_fr(n, seed=args.seed, stream="ntt-2^16")
+    d = pk.DeviceBuffer.from_array(v)
+    for _ in range(args.warmup):
+        pk.ntt_nr_device(d, n, n, 1, s)
+    hip.sync(s)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, s)
+    for _ in range(args.steps):
+        pk.ntt_nr_device(d, n, n, 1, s)
+    hip.record(e1, s)
+    hip.sync(s)
+    ms = hip.elapsed_ms(e0, e1) / args.steps
+    # parity of one fresh transform against the oracle
+    d.upload(v)
+    pk.ntt_nr_device(d, n, n, 1, s)
+    hip.sync(s)
+    got = d.download(shape=(n, 4))
+    ref = v.copy()
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        w = v.copy()
+        pyoracle.ntt_nr(w, n, 1)
+        reps += 1
+        if time.perf_counter() - t0 > min(args.cpu_budget, 5.0):
+            break
+    cpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    pyoracle.ntt_nr(ref, n, 1)
+    modmul = (n // 2) * 16
+    return base_line("Fr elements/s transformed (forward NTT 2^16)", n / (ms / 1e3), "Fr elements/s",
+                     args.steps, args.warmup, ms, True, "u32",
+                     {"workload": "forward NTT ntt_nr, 2^16 Fr, natural -> bit-reversed"},
+                     {"modmul_rate_G": modmul / (ms / 1e3) / 1e9, "matches_oracle": bool(np.array_equal(got, ref)),
+                      "cpu_baseline": {"value": n / (cpu_ms / 1e3), "unit": "Fr elements/s",
+                                       "cores": pyoracle.threads(), "kind": "port",
+                                       "sample": f"{reps} transforms of 2^16, mean {cpu_ms:.2f} ms"}})
+
+
+def workload_p2merkle(pk, hip, s, args):
+    """Poseidon2 2-to-1 Merkle tree over 2^24 uniform Fr leaves, the
+    level launches replayed from one captured hipGraph."""
+    from oracle import pyoracle
+
+    n = 1 << 24
+    leaves = pool_fr(n, args.seed, "p2-leaves")
+    d_leaves = pk.DeviceBuffer.from_array(leaves)
+    d_nodes = pk.DeviceBuffer(32 * (2 * n - 1))
+    g = pk.poseidon2_merkle_graph(d_leaves, n, d_nodes)
+    for _ in range(max(1, args.warmup)):
+        g.launch(s)
+    hip.sync(s)
+    steps = max(1, min(args.steps, 5))
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, s)
+    for _ in range(steps):
+        g.launch(s)
+    hip.record(e1, s)
+    hip.sync(s)
+    ms = hip.elapsed_ms(e0, e1) / steps
+    root = d_nodes.download(np.uint64, nbytes=32 * (2 * n - 1))[-4:]
+    # CPU: a 2^16-leaf subtree (bounded sample), and parity of that subtree's root
+    sub = 1 << 16
+    t0 = time.perf_counter()
+    nodes = pyoracle.p2_merkle_levels(leaves[:sub])
+    cpu_s = time.perf_counter() - t0
+    sub_gpu_root, _ = pk.poseidon2_merkle(leaves[:sub], want_nodes=False)
+    perms = n - 1
+    return base_line("Poseidon2 hashes/s (2-to-1 Merkle over 2^24 Fr leaves, hipGraph)", perms / (ms / 1e3),
+                     "hashes/s", steps, args.warmup, ms, True, "u32",
+                     {"workload": "Poseidon2 t=4 2-to-1 Merkle tree over 2^24 Fr leaves, one captured hipGraph"},
+                     {"modmul_rate_G": perms * 488 / (ms / 1e3) / 1e9,
+                      "root": np.asarray(root, dtype=np.uint64).tobytes().hex(),
+                      "subtree_2p16_matches_oracle": bool(np.array_equal(sub_gpu_root, nodes[-1])),
+                      "cpu_baseline": {"value": (sub - 1) / cpu_s, "unit": "hashes/s",
+                                       "cores": pyoracle.threads(), "kind": "port",
+                                       "sample": f"2^16-leaf subtree ({sub - 1} permutations) in {cpu_s:.2f} s"}})
+
+
+def csr_template(rng, n, vals_pool):
+    k = rng.integers(1, 5, size=n)
+    cand = np.sort(rng.integers(0, n, size=(n, 4)), axis=1)
+    keep = np.ones((n, 4), bool)
+    keep[:, 1:] = cand[:, 1:] != cand[:, :-1]
+    keep &= np.arange(4)[None, :] < k[:, None]
+    rs = np.zeros(n, np.uint32)
+    rs[1:] = np.cumsum(keep.sum(axis=1))[:-1]
+    cols = cand[keep].astype(np.uint32)
+    off = int(rng.integers(0, vals_pool.shape[0]))
+    reps = (cols.shape[0] + vals_pool.shape[0] - 1) // vals_pool.shape[0] + 1
+    vals = np.ascontiguousarray(np.tile(np.roll(vals_pool, -off, axis=0), (reps, 1))[: cols.shape[0]])
+    return rs, cols, vals
+
+
+def workload_serving(pk, hip, s, args):
+    """Serving: Poisson stream of R1CS jobs, m ~ U{16..22}, 1..4 nonzeros per
+    row; each job: Az, Bz, Cz, the full sumcheck, the witness commit."""
+    rng = np.random.default_rng(args.seed)
+    pool = pool_fr(1 << 20, args.seed, "serve-pool")
+    templates = {}
+    for m in range(16, 23):
+        n = 1 << m
+        mats = []
+        for _ in range(3):
+            rs, cols, vals = csr_template(rng, n, pool)
+            mats.append(tuple(pk.DeviceBuffer.from_array(x) for x in (rs, cols, vals)))
+        z = np.ascontiguousarray(np.tile(pool, ((n + pool.shape[0] - 1) // pool.shape[0], 1))[:n])
+        z[0] = np.array([0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E,
+                         0x0E0A77C19A07DF2F], dtype=np.uint64)  # z[0] = 1
+        templates[m] = (mats, pk.DeviceBuffer.from_array(z))
+    jobs = args.jobs
+    ms_ = rng.integers(16, 23, size=jobs)
+    rate = args.rate
+    gaps = rng.exponential(1.0 / rate, size=jobs)
+    srv = pk.Server(hbm_budget_bytes=args.budget_gb << 30, slots=args.slots)
+    # warm-up: one job of each size
+    for m in range(16, 23):
+        mats, z = templates[m]
+        j = srv.submit(m, mats, z, seed=1)
+        srv.wait(j, timeout_ms=120000)
+        srv.release(j)
+    t0 = time.perf_counter()
+    outstanding, results = [], []
+
+    def reap(block):
+        # completed jobs leave in arrival order; their resident codeword/tree is
+        # released once the result is read (the openings would come before that)
+        while outstanding and (block or srv.poll(outstanding[0])):
+            j = outstanding.pop(0)
+            results.append(srv.wait(j, timeout_ms=120000))
+            srv.release(j)
+
+    t_next = t0
+    for i in range(jobs):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            reap(False)
+            time.sleep(min(0.0002, max(0.0, t_next - time.perf_counter())))
+        mats, z = templates[int(ms_[i])]
+        outstanding.append(srv.submit(int(ms_[i]), mats, z, seed=1000 + i))
+    reap(True)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.destroy()
+    lat = [r["latency_ms"] for r in results]
+    elems = int(sum(1 << int(m) for m in ms_))
+    return base_line("jobs/s (R1CS SpMV + sumcheck + witness commit, Poisson arrivals)", jobs / wall,
+                     "jobs/s", jobs, 7, wall * 1e3 / jobs, True, "u32",
+                     {"workload": f"serving: {jobs} jobs, m~U{{16..22}}, 1-4 nnz/row, Poisson rate {rate}/s, "
+                                  f"{args.slots} slots, HBM budget {args.budget_gb} GiB",
+                      "rate_per_s": rate},
+                     {"p50_job_ms": pct(lat, 0.5), "p99_job_ms": pct(lat, 0.99),
+                      "mean_queue_ms": statistics.mean(r["queue_ms"] for r in results),
+                      "witness_elements_per_s": elems / wall, "peak_in_flight": st["peak_in_flight"]})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,11 +406,20 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a 256 MiB device-to-device copy kernel (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", default="commit",
+                    choices=["commit", "ntt", "p2merkle", "serving"],
+                    help="commit (default: 2^20-coefficient Skyscraper commit) | ntt (forward NTT 2^16) | p2merkle (Poseidon2 tree over 2^24 leaves) | serving (Poisson R1CS jobs)")
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=60.0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=int, default=48)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+              "(the multi-GPU sharding is a design, not a run).", file=sys.stderr)
         sys.exit(2)
 
     import provekit_amd as pk
@@ -236,6 +428,10 @@ def main():
     pk.init(0)
     hip = Hip()
     s = hip.stream()
+    if args.workload != "commit":
+        fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving}[args.workload]
+        print(json.dumps(fn(pk, hip, s, args)), flush=True)
+        return
 
     n = 1 << args.log_n
     N = n * args.blowup
@@ -351,6 +547,31 @@ def main():
         out["cpu_baseline"] = cb
         out["root_matches_cpu"] = croot == root
     print(json.dumps(out), flush=True)
+    if args.copy_probe:
+        copy_probe()
+
+
+def copy_probe(nbytes=256 << 20, reps=3):
+    """A plain copy of known size: its FETCH_SIZE/WRITE_SIZE under rocprofv3
+    --pmc give the counters' bytes-per-unit on this device."""
+    import torch
+
+    import provekit_amd as pk
+
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda:0").random_()
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    # the library's own element copy (32-byte elements as two 16-byte lanes
+    # loads, the access pattern of the NTT and hashing kernels): one-chunk
+    # hash_many is a copy
+    src = pk.DeviceBuffer(nbytes)
+    dst = pk.DeviceBuffer(nbytes)
+    for _ in range(reps):
+        pk.hash_many_device(32, src, nbytes // 32, dst)
+    pk.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
 if __name__ == "__main__":
