@@ -76,6 +76,21 @@ int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size
 int pkgpu_intt_rn(uint64_t* values, size_t len);
 int pkgpu_intt_rn_device(uint64_t* values, size_t len, void* stream);
 
+/* NTT schedule tuning (no counterpart in the reference, whose ntt crate runs
+ * one fixed schedule, ntt/src/ntt.rs:150-248).  Every transform above picks
+ * its pass shape (tile of 2^tile_log elements in LDS, butterflies per thread,
+ * minimum consecutive run 2^min_run in early passes) from a table keyed by
+ * (log2 codeword, stages run).  The table is read when the library starts
+ * from $PKGPU_NTT_TUNING, else from ntt_tuning.json beside the library;
+ * missing lengths take the nearest tuned one, an empty table tiles of 2^11.
+ * autotune times every shape for log2 codeword in [log_min, log_max] (1..27)
+ * and 0..max_skip skipped stages (blowup 2^max_skip), keeps the fastest and,
+ * if save_path is given, writes the table there. */
+int pkgpu_ntt_autotune(int log_min, int log_max, int max_skip, const char* save_path,
+                       double* seconds);
+int pkgpu_ntt_tuning_load(const char* path);
+int pkgpu_ntt_tuning_get(int log_codeword, int stages, int* tile_log, int* bpt, int* min_run);
+
 /* ------------------------------------------------------------ RSFr plugin */
 /* ReedSolomon<Fr> for RSFr (provekit/common/src/ntt.rs:12-105). */
 /* next_order: PKGPU_ERR_SIZE where the reference returns None (> 2^28). */
@@ -211,7 +226,8 @@ int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int cos
  * Polynomial i: coeffs[i] (n[i] Fr, power of two), blowup[i] (power of two),
  * coset[i] (0: H, else 5*H), codewords[i] (n[i]*blowup[i] Fr),
  * nodes[i] (2*leaves-1 digests, leaves = n[i]*blowup[i]/leaf_elems).
- * tile_log (9..11): NTT tile of 2^tile_log elements (0: the tuned default). */
+ * tile_log: 0 takes each polynomial's tuned NTT shape; 6..11 forces tiles
+ * of 2^tile_log elements with the default shape. */
 int pkgpu_batch_plan_create(size_t count, const uint64_t* const* coeffs, const size_t* n,
                             const size_t* blowup, const int* coset, size_t leaf_elems,
                             uint64_t* const* codewords, void* const* nodes, int tile_log,

@@ -82,6 +82,9 @@ SIGNATURES = {
     "pkgpu_eq_evaluations_device": [_vp, _sz, _sz, _vp, _vp],
     "pkgpu_sumcheck_r1cs_round_device": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp],
     "pkgpu_sumcheck_prove_device": [_vp, _vp, _vp, _sz, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_ntt_autotune": [_i, _i, _i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)],
+    "pkgpu_ntt_tuning_load": [ctypes.c_char_p],
+    "pkgpu_ntt_tuning_get": [_i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "pkgpu_batch_plan_create": [_sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _i, ctypes.POINTER(_vp)],
     "pkgpu_batch_run": [_vp, _vp],
     "pkgpu_batch_graph_launch": [_vp, _vp],
@@ -166,6 +169,26 @@ def ntt_nr(values: np.ndarray, codeword_size: int, num_groups: int) -> None:
     if values.dtype != np.uint64 or not values.flags.c_contiguous:
         raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
     _call("pkgpu_ntt_nr", _ptr(values), values.size // 4, codeword_size, num_groups)
+
+
+def ntt_autotune(log_min: int, log_max: int, max_skip: int = 2, save_path: Optional[str] = None) -> float:
+    """Time every NTT pass shape for the given lengths on the current device
+    and keep the fastest per (length, stages); returns the seconds spent."""
+    _ensure_init()
+    t = ctypes.c_double()
+    _call("pkgpu_ntt_autotune", log_min, log_max, max_skip,
+          save_path.encode() if save_path else None, ctypes.byref(t))
+    return t.value
+
+
+def ntt_tuning_load(path: str) -> None:
+    _call("pkgpu_ntt_tuning_load", path.encode())
+
+
+def ntt_tuning_get(log_codeword: int, stages: int) -> dict:
+    a, b, c = _i(), _i(), _i()
+    _call("pkgpu_ntt_tuning_get", log_codeword, stages, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return {"tile_log": a.value, "bpt": b.value, "min_run": c.value}
 
 
 def intt_rn(values: np.ndarray) -> None:

@@ -299,6 +299,35 @@ int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size
                    num_groups, stream_or_thread(stream));
   });
 }
+int pkgpu_ntt_autotune(int log_min, int log_max, int max_skip, const char* save_path,
+                       double* seconds) {
+  return guarded([&] {
+    REQ(log_min >= 1 && log_max <= 27 && log_min <= log_max, pk::PK_ERR_ARG,
+        "ntt autotune: log range must lie in [1, 27]");
+    REQ(max_skip >= 0 && max_skip <= 8, pk::PK_ERR_ARG, "ntt autotune: max_skip must be 0..8");
+    hipStream_t s = pk::thread_stream();
+    const double t = pk::ntt_autotune(pk::device_current(), log_min, log_max, max_skip, s);
+    if (save_path) pk::ntt_tuning_save(save_path);
+    if (seconds) *seconds = t;
+  });
+}
+int pkgpu_ntt_tuning_load(const char* path) {
+  return guarded([&] {
+    REQ(path, pk::PK_ERR_ARG, "ntt tuning: null path");
+    pk::ntt_tuning_load(path);
+  });
+}
+int pkgpu_ntt_tuning_get(int log_codeword, int stages, int* tile_log, int* bpt, int* min_run) {
+  return guarded([&] {
+    REQ(tile_log && bpt && min_run, pk::PK_ERR_ARG, "ntt tuning: null output");
+    REQ(log_codeword >= 0 && stages >= 0 && stages <= log_codeword, pk::PK_ERR_ARG,
+        "ntt tuning: stages must lie in [0, log_codeword]");
+    const pk::NttShape sh = pk::ntt_tuned_shape(log_codeword, stages);
+    *tile_log = sh.tile_log;
+    *bpt = sh.bpt ? sh.bpt : pk::ntt_default_bpt(sh.tile_log);
+    *min_run = sh.min_run < 0 ? 2 : sh.min_run;
+  });
+}
 int pkgpu_intt_rn(uint64_t* values, size_t len) {
   return guarded([&] {
     REQ(len == 0 || values, pk::PK_ERR_ARG, "intt_rn: null values");
@@ -727,7 +756,7 @@ int pkgpu_batch_plan_create(size_t count, const uint64_t* const* coeffs, const s
       specs[i].codeword = reinterpret_cast<fr_t*>(codewords[i]);
       specs[i].nodes = static_cast<fr_t*>(nodes[i]);
     }
-    *plan = pk::batch_plan_create(pk::device_current(), specs, tile_log ? tile_log : 11);
+    *plan = pk::batch_plan_create(pk::device_current(), specs, tile_log);
   });
 }
 int pkgpu_batch_run(void* plan, void* stream) {

@@ -156,7 +156,8 @@ BatchPlan::~BatchPlan() {
 
 BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs, int tile_log) {
   PK_REQUIRE(!specs.empty(), PK_ERR_ARG, "batch: no polynomials");
-  PK_REQUIRE(tile_log >= 9 && tile_log <= 11, PK_ERR_ARG, "batch: tile log must be 9..11");
+  PK_REQUIRE(tile_log == 0 || (tile_log >= 6 && tile_log <= 11), PK_ERR_ARG,
+             "batch: tile log must be 0 (tuned) or 6..11");
   auto plan = std::make_unique<BatchPlan>();
   plan->count = (uint32_t)specs.size();
   std::vector<BatchPolyDev> polys(specs.size());
@@ -200,28 +201,38 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
       chunk_off += s.n;
     }
     const fr_t* src = s.coset ? p.chunk : s.coeffs;
-    nplans[i] = ntt_plan(d, s.codeword, p.N, p.N, s.blowup, src, s.n, tile_log);
+    NttShape shape;
+    shape.tile_log = tile_log;  // 0: the tuned shape of this length
+    nplans[i] = ntt_plan(d, s.codeword, p.N, p.N, s.blowup, src, s.n, shape);
     PK_REQUIRE(!nplans[i].empty(), PK_ERR_SIZE, "batch: blowup must be below the codeword size");
     max_rounds = std::max(max_rounds, nplans[i].size());
   }
   plan->polys = upload_vec(polys, plan->owned);
   plan->coset_map = upload_vec(coset_map, plan->owned);
   plan->coset_blocks = (uint32_t)coset_map.size();
-  // pass rounds, grouped by tile size
+  // pass rounds, grouped by kernel shape (tile size, butterflies per thread)
   std::vector<NttPassDesc> descs;
   for (size_t r = 0; r < max_rounds; r++) {
-    for (int tl = 1; tl <= 11; tl++) {
+    std::vector<std::pair<int, int>> shapes;
+    for (size_t i = 0; i < specs.size(); i++) {
+      if (r >= nplans[i].size()) continue;
+      const std::pair<int, int> sh(nplans[i][r].tl, nplans[i][r].bpt);
+      if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
+    }
+    std::sort(shapes.begin(), shapes.end());
+    for (const auto& sh : shapes) {
       std::vector<uint2> map;
       for (size_t i = 0; i < specs.size(); i++) {
-        if (r >= nplans[i].size() || nplans[i][r].tl != tl) continue;
+        if (r >= nplans[i].size() || nplans[i][r].tl != sh.first || nplans[i][r].bpt != sh.second)
+          continue;
         const uint32_t di = (uint32_t)descs.size();
         descs.push_back(nplans[i][r]);
-        const uint64_t tiles = polys[i].N >> tl;
+        const uint64_t tiles = polys[i].N >> sh.first;
         for (uint64_t t = 0; t < tiles; t++) map.push_back(make_uint2(di, (uint32_t)t));
       }
-      if (map.empty()) continue;
       BatchPlan::Round rd;
-      rd.tl = tl;
+      rd.tl = sh.first;
+      rd.bpt = sh.second;
       rd.blocks = (uint32_t)map.size();
       rd.map = upload_vec(map, plan->owned);
       plan->rounds.push_back(rd);
@@ -286,7 +297,7 @@ void batch_run(BatchPlan& plan, hipStream_t s) {
   if (plan.coset_blocks)
     k_coset_scale_batch<<<plan.coset_blocks, 256, 0, s>>>(plan.polys, plan.coset_map, plan.coset);
   PK_HIP(hipGetLastError());
-  for (const BatchPlan::Round& rd : plan.rounds) ntt_batch_round_dev(plan.descs, rd.map, rd.blocks, rd.tl, s);
+  for (const BatchPlan::Round& rd : plan.rounds) ntt_batch_round_dev(plan.descs, rd.map, rd.blocks, rd.tl, rd.bpt, s);
   k_hash_queue<<<plan.queue_blocks, kQueueWave, 0, s>>>(plan.polys, plan.tasks, plan.ntasks,
                                                          plan.done, plan.next, plan.err);
   k_gather_roots<<<(plan.count + 63) / 64, 64, 0, s>>>(plan.polys, plan.count, plan.roots);

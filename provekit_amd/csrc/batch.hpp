@@ -34,7 +34,7 @@ struct HashTask {
 
 struct BatchPlan {
   struct Round {
-    int tl;
+    int tl, bpt;
     uint32_t blocks;
     const uint2* map;
   };

@@ -7,8 +7,10 @@
 // Value ranges used by the kernels:
 //   canonical  [0, p)
 //   lazy       [0, 2p)   (4p < 2^256, so sums of two lazy values never wrap)
+//   NTT lazy   [0, 4p)   (between butterfly stages only)
 // Montgomery products of two lazy values are lazy without a final subtraction
-// because a*b/2^256 + p < 4p^2/2^256 + p < 2p.
+// because a*b/2^256 + p < 4p^2/2^256 + p < 2p; the same holds for a canonical
+// times a value below 4p (4p^2/2^256 + p < 1.76p).
 //
 // Multiplication is Montgomery with 32-bit digits in product-scanning form:
 // every digit product is one v_mad_u64_u32 (32x32 + 64 -> 64) plus one
@@ -100,6 +102,26 @@ __device__ __forceinline__ fr_t fr_sub(const fr_t& a, const fr_t& b) {
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) r.l[i] = addc(r.l[i], p2_limb(i) & mask, c);
+  return r;
+}
+// a + b without reduction (caller guarantees a + b < 2^256)
+__device__ __forceinline__ fr_t fr_add_raw(const fr_t& a, const fr_t& b) {
+  fr_t r;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.l[i] = addc(a.l[i], b.l[i], c);
+  return r;
+}
+// a - b + 2p for a, b in [0, 2p): the result lies in (0, 4p), so the wrap of
+// the subtraction and the carry of the addition cancel modulo 2^256.
+__device__ __forceinline__ fr_t fr_sub_plus_2p(const fr_t& a, const fr_t& b) {
+  fr_t r;
+  uint32_t br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.l[i] = subb(a.l[i], b.l[i], br);
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.l[i] = addc(r.l[i], p2_limb(i), c);
   return r;
 }
 // canonical + canonical -> canonical

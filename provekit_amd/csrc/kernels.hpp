@@ -7,22 +7,45 @@ namespace pk {
 
 // ---- NTT (ntt.hip) ----
 // One NTT pass over a tile grid: stages [s0, s0 + r) of codeword index bits,
-// tiles of 2^tl = 2^(r + jb) elements (see k_ntt_pass).
+// tiles of 2^tl = 2^(r + jb) elements, bpt butterflies per thread per stage
+// (see k_ntt_pass).
 struct NttPassDesc {
   fr_t* data;
   const fr_t* src;  // first pass only: read element g from src[g % chunk]
   const fr_t* rr;   // root table
   uint64_t chunk;
   uint64_t Q;       // low stride: 2^q * M elements below the window
-  int r, jb, canon, tl;
+  int r, jb, canon, tl, bpt;
 };
-// The passes of ntt_nr over `data` with tiles of at most 2^tile_log elements.
+// Schedule knobs of one transform: tiles of at most 2^tile_log elements,
+// bpt butterflies per thread, earlier passes keep runs of >= 2^min_run
+// consecutive elements.  tile_log 0 takes the tuned shape for the length.
+struct NttShape {
+  int tile_log = 0;
+  int bpt = 0;       // 0: the tile's default
+  int min_run = -1;  // -1: 2
+};
+bool ntt_shape_supported(int tile_log, int bpt);
+int ntt_default_bpt(int tile_log);
+// The passes of ntt_nr over `data`.
 std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t codeword,
                                   uint64_t num_groups, const fr_t* src, uint64_t chunk,
-                                  int tile_log);
+                                  NttShape shape = NttShape());
+void ntt_run_plan(const std::vector<NttPassDesc>& plan, uint64_t len, hipStream_t s);
 // Run one batched pass round: workgroup b runs tile map[b].y of descs[map[b].x].
 void ntt_batch_round_dev(const NttPassDesc* descs, const uint2* map, uint32_t blocks, int tl,
-                         hipStream_t s);
+                         int bpt, hipStream_t s);
+// Tuned shapes (tune.cpp): keyed by (log2 codeword, stages run).  The table
+// is read once from the tuning file (PKGPU_NTT_TUNING, else ntt_tuning.json
+// beside the library); lengths without an entry use the nearest tuned one
+// with the same number of skipped stages, else tiles of 2^11.
+NttShape ntt_tuned_shape(int log_codeword, int stages);
+void ntt_tuning_put(int log_codeword, int stages, NttShape shape, double us);
+void ntt_tuning_load(const std::string& path);
+void ntt_tuning_save(const std::string& path);
+// Times every supported shape for log_codeword in [log_min, log_max] and
+// skipped stages 0..max_skip on `s`, keeps the fastest; returns seconds spent.
+double ntt_autotune(Device& d, int log_min, int log_max, int max_skip, hipStream_t s);
 // In-place DIT NTT, natural -> bit-reversed order, over `len` elements of
 // `len / codeword` interleaved polynomials, starting at stage log2(num_groups)
 // (ntt::ntt_nr).  If `src` is given, the first pass reads element g from

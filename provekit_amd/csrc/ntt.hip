@@ -77,11 +77,15 @@ void build_power_table(fr_t* out, const hfr& base_h, int log, bool bit_reversed,
 
 using PassArgs = NttPassDesc;
 
-template <int TL>
+// A pass kernel is instantiated per tile size 2^TL and butterflies per
+// thread per stage BPT: 2^(TL-1) / BPT threads.  BPT trades occupancy (more
+// waves per tile to hide the per-stage barrier and the tile load) against
+// per-thread independent work; the autotuner picks it per length.
+template <int TL, int BPT>
 struct PassShape {
-  static constexpr int T = (TL >= 3) ? ((1 << (TL - 3)) > 256 ? 256 : (1 << (TL - 3))) : 1;
-  static constexpr int BF = (TL >= 1) ? (1 << (TL - 1)) : 0;  // butterflies per stage
-  static constexpr int BPT = BF / T;
+  static constexpr int BF = 1 << (TL - 1);  // butterflies per stage
+  static constexpr int T = BF / BPT;
+  static_assert(BF % BPT == 0 && T >= 1 && T <= 1024, "pass shape");
 };
 
 // Flat element index of codeword position i, column c: i * M + c.  A pass
@@ -89,10 +93,13 @@ struct PassShape {
 // and i_lo the q bits below; the element sits at
 //   h * (2^r * Q) + w * Q + l,   l = i_lo * M + c in [0, Q).
 // A tile is one h, all 2^r windows, and 2^jb consecutive l (2^jb divides Q).
-template <int TL>
+//
+// Butterfly (Harvey's lazy form): values live in [0, 4p) between stages.
+// x is first brought below 2p, the twiddle product m = w * y is below 2p for
+// y < 4p (w canonical), and x + m, x - m + 2p stay below 4p < 2^256.
+template <int TL, int BPT>
 __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, fr_t* lds) {
-  constexpr int T = PassShape<TL>::T;
-  constexpr int BPT = PassShape<TL>::BPT;
+  constexpr int T = PassShape<TL, BPT>::T;
   constexpr int N = 1 << TL;
   const int tid = threadIdx.x;
   const int jb = a.jb;
@@ -130,8 +137,10 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
       const fr_t m = fr_mul(tw[k], y[k]);
-      lds[e0[k]] = fr_add(x[k], m);
-      lds[e0[k] + span] = fr_sub(x[k], m);
+      fr_t u = x[k];
+      cond_sub<2>(u);
+      lds[e0[k]] = fr_add_raw(u, m);
+      lds[e0[k] + span] = fr_sub_plus_2p(u, m);
     }
     __syncthreads();
   }
@@ -139,73 +148,55 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   for (int e = tid; e < N; e += T) {
     const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
     fr_t v = lds[e];
-    if (a.canon) fr_canon(v);
+    if (a.canon) {
+      cond_sub<2>(v);
+      fr_canon(v);
+    }
     fr_store(a.data + g, v);
   }
 }
 
-template <int TL>
-__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass(PassArgs a) {
+template <int TL, int BPT>
+__global__ __launch_bounds__((PassShape<TL, BPT>::T)) void k_ntt_pass(PassArgs a) {
   __shared__ fr_t lds[1 << TL];
-  ntt_pass_tile<TL>(a, blockIdx.x, lds);
+  ntt_pass_tile<TL, BPT>(a, blockIdx.x, lds);
 }
 
 // Batched pass: workgroup b runs tile map[b].y of descriptor map[b].x, so one
-// launch advances many polynomials' NTTs (all with tiles of 2^TL elements).
-template <int TL>
-__global__ __launch_bounds__(PassShape<TL>::T) void k_ntt_pass_batch(const PassArgs* descs,
-                                                                     const uint2* map) {
+// launch advances many polynomials' NTTs (all with the same kernel shape).
+template <int TL, int BPT>
+__global__ __launch_bounds__((PassShape<TL, BPT>::T)) void k_ntt_pass_batch(const PassArgs* descs,
+                                                                          const uint2* map) {
   __shared__ fr_t lds[1 << TL];
   const uint2 m = map[blockIdx.x];
-  ntt_pass_tile<TL>(descs[m.x], m.y, lds);
+  ntt_pass_tile<TL, BPT>(descs[m.x], m.y, lds);
 }
 
-template <int TL>
-void launch_pass(const PassArgs& a, uint64_t len, hipStream_t s) {
-  const uint64_t tiles = len >> TL;
-  k_ntt_pass<TL><<<(unsigned)tiles, PassShape<TL>::T, 0, s>>>(a);
+struct PassVariant {
+  int tl, bpt, threads;
+  void (*plain)(PassArgs);
+  void (*batch)(const PassArgs*, const uint2*);
+};
+
+template <int TL, int BPT>
+constexpr PassVariant variant() {
+  return {TL, BPT, PassShape<TL, BPT>::T, k_ntt_pass<TL, BPT>, k_ntt_pass_batch<TL, BPT>};
 }
 
-void launch_pass_tl(int tl, const PassArgs& a, uint64_t len, hipStream_t s) {
-  switch (tl) {
-    case 1: launch_pass<1>(a, len, s); break;
-    case 2: launch_pass<2>(a, len, s); break;
-    case 3: launch_pass<3>(a, len, s); break;
-    case 4: launch_pass<4>(a, len, s); break;
-    case 5: launch_pass<5>(a, len, s); break;
-    case 6: launch_pass<6>(a, len, s); break;
-    case 7: launch_pass<7>(a, len, s); break;
-    case 8: launch_pass<8>(a, len, s); break;
-    case 9: launch_pass<9>(a, len, s); break;
-    case 10: launch_pass<10>(a, len, s); break;
-    case 11: launch_pass<11>(a, len, s); break;
-    default: throw Error(PK_ERR_INTERNAL, "ntt: bad tile size");
-  }
-  PK_HIP(hipGetLastError());
-}
+// Small tiles have one shape; tiles of 2^6..2^11 come with 1, 2 or 4
+// butterflies per thread.
+const PassVariant kVariants[] = {
+    variant<1, 1>(),  variant<2, 2>(),  variant<3, 4>(),  variant<4, 4>(),  variant<5, 4>(),
+    variant<6, 1>(),  variant<6, 2>(),  variant<6, 4>(),  variant<7, 1>(),  variant<7, 2>(),
+    variant<7, 4>(),  variant<8, 1>(),  variant<8, 2>(),  variant<8, 4>(),  variant<9, 1>(),
+    variant<9, 2>(),  variant<9, 4>(),  variant<10, 1>(), variant<10, 2>(), variant<10, 4>(),
+    variant<11, 1>(), variant<11, 2>(), variant<11, 4>(),
+};
 
-template <int TL>
-void launch_batch(const PassArgs* descs, const uint2* map, uint32_t blocks, hipStream_t s) {
-  k_ntt_pass_batch<TL><<<blocks, PassShape<TL>::T, 0, s>>>(descs, map);
-}
-
-void launch_pass_batch_tl(int tl, const PassArgs* descs, const uint2* map, uint32_t blocks,
-                          hipStream_t s) {
-  switch (tl) {
-    case 1: launch_batch<1>(descs, map, blocks, s); break;
-    case 2: launch_batch<2>(descs, map, blocks, s); break;
-    case 3: launch_batch<3>(descs, map, blocks, s); break;
-    case 4: launch_batch<4>(descs, map, blocks, s); break;
-    case 5: launch_batch<5>(descs, map, blocks, s); break;
-    case 6: launch_batch<6>(descs, map, blocks, s); break;
-    case 7: launch_batch<7>(descs, map, blocks, s); break;
-    case 8: launch_batch<8>(descs, map, blocks, s); break;
-    case 9: launch_batch<9>(descs, map, blocks, s); break;
-    case 10: launch_batch<10>(descs, map, blocks, s); break;
-    case 11: launch_batch<11>(descs, map, blocks, s); break;
-    default: throw Error(PK_ERR_INTERNAL, "ntt batch: bad tile size");
-  }
-  PK_HIP(hipGetLastError());
+const PassVariant& find_variant(int tl, int bpt) {
+  for (const PassVariant& v : kVariants)
+    if (v.tl == tl && v.bpt == bpt) return v;
+  throw Error(PK_ERR_INTERNAL, "ntt: no pass kernel for this tile shape");
 }
 
 // Copy with replication: data[g] = src[g % chunk].
@@ -216,40 +207,51 @@ __global__ void k_replicate(fr_t* data, const fr_t* src, uint64_t chunk, uint64_
 
 }  // namespace
 
+bool ntt_shape_supported(int tile_log, int bpt) {
+  for (const PassVariant& v : kVariants)
+    if (v.tl == tile_log && v.bpt == bpt) return true;
+  return false;
+}
+
+int ntt_default_bpt(int tile_log) {
+  return tile_log >= 3 ? 4 : (tile_log == 2 ? 2 : 1);
+}
+
 struct NttPass {
   int s0, r, jb;
 };
 
 // Pass plan for stages [s_begin, s_end) of codeword index bits (L = s_end)
-// with interleave width M.  The last pass takes as many stages as fit a full
-// tile; earlier passes split the rest evenly, each with at least 2^kMinRun
-// consecutive elements per run where the layout allows.  jb is capped by the
-// power of two dividing the low stride 2^q * M.
-static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_twos, int kTileLog) {
-  constexpr int kMinRun = 2;
+// with interleave width M = 2^m_twos * odd.  The last pass takes as many
+// stages as fit a full tile; earlier passes split the rest evenly, each
+// keeping at least 2^min_run consecutive elements per run where the layout
+// allows.  jb is capped by the power of two dividing the low stride 2^q * M.
+static std::vector<NttPass> plan_passes(int L, int s_begin, int s_end, int m_twos, int tile_log,
+                                        int min_run) {
   std::vector<NttPass> out;
   int rem = s_end - s_begin;
   if (rem <= 0) return out;
-  int r_fin = rem < kTileLog ? rem : kTileLog;
+  int r_fin = rem < tile_log ? rem : tile_log;
   int rest = rem - r_fin;
-  int nfirst = (rest + (kTileLog - kMinRun) - 1) / (kTileLog - kMinRun);
+  const int per = tile_log - min_run;
+  int nfirst = (rest + per - 1) / per;
   int s0 = s_begin;
   for (int i = 0; i < nfirst; i++) {
     int r = rest / nfirst + (i < rest % nfirst ? 1 : 0);
     int q2 = (L - s0 - r) + m_twos;
-    int jb = q2 < (kTileLog - r) ? q2 : (kTileLog - r);
+    int jb = q2 < (tile_log - r) ? q2 : (tile_log - r);
     out.push_back({s0, r, jb});
     s0 += r;
   }
   int q2 = (L - s_end) + m_twos;
-  int jb = q2 < (kTileLog - r_fin) ? q2 : (kTileLog - r_fin);
+  int jb = q2 < (tile_log - r_fin) ? q2 : (tile_log - r_fin);
   out.push_back({s0, r_fin, jb});
   return out;
 }
 
 std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t codeword,
                                   uint64_t num_groups, const fr_t* src, uint64_t chunk,
-                                  int tile_log) {
+                                  NttShape shape) {
   PK_REQUIRE(codeword >= 1 && is_pow2(codeword) && len % codeword == 0, PK_ERR_SIZE,
              "ntt: length must be a multiple of the power-of-two codeword size");
   PK_REQUIRE(is_pow2(num_groups) && num_groups <= codeword, PK_ERR_SIZE,
@@ -259,7 +261,12 @@ std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t 
   while (((M >> m_twos) & 1) == 0) m_twos++;
   const int L = log2_exact(codeword);
   const int s_begin = log2_exact(num_groups);
-  std::vector<NttPass> plan = plan_passes(L, s_begin, L, m_twos, tile_log);
+  if (shape.tile_log == 0) shape = ntt_tuned_shape(L, L - s_begin);
+  if (shape.bpt == 0) shape.bpt = ntt_default_bpt(shape.tile_log);
+  if (shape.min_run < 0) shape.min_run = 2;
+  PK_REQUIRE(ntt_shape_supported(shape.tile_log, shape.bpt) && shape.min_run < shape.tile_log,
+             PK_ERR_ARG, "ntt: unsupported tile shape");
+  std::vector<NttPass> plan = plan_passes(L, s_begin, L, m_twos, shape.tile_log, shape.min_run);
   std::vector<NttPassDesc> out;
   if (plan.empty()) return out;
   const fr_t* rr = roots_table(d, L - 1);
@@ -274,14 +281,26 @@ std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t 
     a.jb = plan[i].jb;
     a.canon = (i + 1 == plan.size());
     a.tl = plan[i].r + plan[i].jb;
+    // a pass narrower than the tuned tile (short transforms) takes its
+    // tile's default shape
+    a.bpt = a.tl == shape.tile_log ? shape.bpt : ntt_default_bpt(a.tl);
+    if (!ntt_shape_supported(a.tl, a.bpt)) a.bpt = ntt_default_bpt(a.tl);
     out.push_back(a);
   }
   return out;
 }
 
+void ntt_run_plan(const std::vector<NttPassDesc>& plan, uint64_t len, hipStream_t s) {
+  for (const NttPassDesc& a : plan) {
+    const PassVariant& v = find_variant(a.tl, a.bpt);
+    v.plain<<<(unsigned)(len >> a.tl), v.threads, 0, s>>>(a);
+    PK_HIP(hipGetLastError());
+  }
+}
+
 void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t num_groups,
                 hipStream_t s, const fr_t* src, uint64_t chunk) {
-  std::vector<NttPassDesc> plan = ntt_plan(d, data, len, codeword, num_groups, src, chunk, 11);
+  std::vector<NttPassDesc> plan = ntt_plan(d, data, len, codeword, num_groups, src, chunk);
   if (plan.empty()) {
     if (src) {
       k_replicate<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(data, src, chunk, len);
@@ -289,12 +308,15 @@ void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t
     }
     return;
   }
-  for (const NttPassDesc& a : plan) launch_pass_tl(a.tl, a, len, s);
+  ntt_run_plan(plan, len, s);
 }
 
 void ntt_batch_round_dev(const NttPassDesc* descs, const uint2* map, uint32_t blocks, int tl,
-                         hipStream_t s) {
-  if (blocks) launch_pass_batch_tl(tl, descs, map, blocks, s);
+                         int bpt, hipStream_t s) {
+  if (!blocks) return;
+  const PassVariant& v = find_variant(tl, bpt);
+  v.batch<<<blocks, v.threads, 0, s>>>(descs, map);
+  PK_HIP(hipGetLastError());
 }
 
 namespace {

@@ -12,7 +12,7 @@ def say(msg):
     print(f"[case] {msg}", flush=True)
 
 
-def test_config1_forward_ntt_2p16(gpu, oracle):
+def test_forward_ntt_2p16(gpu, oracle):
     # forward NTT of 2^16 uniform Fr, natural order in, bit-reversed out
     v = random_fr(1 << 16, seed=1, stream="ntt-2^16")
     say("ntt_nr 2^16")

@@ -83,7 +83,7 @@ def test_commit_small(gpu, oracle, log_n, blowup, coset):
     assert root == eroot
 
 
-def test_commit_config2_full_size(gpu, oracle):
+def test_commit_2p20_blowup4_full_size(gpu, oracle):
     # 2^20 coefficients, blowup 4, coset NTT to 2^22, leaves of 16, Skyscraper tree
     c = random_fr(1 << 20, seed=2, stream="commit-2^20")
     say("commit 2^20 -> 2^22 (full size)")
