@@ -49,52 +49,40 @@ __global__ void k_fill_zero_u32(uint32_t* p, uint64_t n) {
   if (i < n) p[i] = 0;
 }
 
-__device__ __forceinline__ bool wait_done(const uint32_t* done, int32_t dep, uint32_t* err) {
-  if (dep < 0) return true;
-  for (uint64_t it = 0; it < kSpinLimit; it++) {
-    if (__hip_atomic_load(done + dep, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return true;
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return false;
-}
-
 // Offset of tree level l (level 0 = n0 leaves) in the level-by-level node
 // array: sum_{i<l} n0 >> i = 2 n0 - 2 (n0 >> l).
 __device__ __forceinline__ uint64_t level_off(uint64_t n0, uint32_t l) {
   return 2 * n0 - 2 * (n0 >> l);
 }
 
-__global__ __launch_bounds__(kQueueWave) void k_hash_queue(const BatchPolyDev* polys,
-                                                           const HashTask* tasks, uint32_t ntasks,
-                                                           uint32_t* done, uint32_t* next,
-                                                           uint32_t* err) {
-  __shared__ uint32_t tid_sh;
-  __shared__ fr_t lvl[kQueueWave];
-  const int t = threadIdx.x;
+// Workgroups of four waves, each wave an independent worker taking tasks:
+// the dispatcher places one wave per SIMD per workgroup (one-wave
+// workgroups can stack several waves on one SIMD while another idles).  No
+// workgroup barrier: broadcasts and the tail's level exchange are wave-level.
+constexpr int kQueueBlock = 256;
+
+__global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* polys,
+                                                            const HashTask* tasks, uint32_t ntasks,
+                                                            uint32_t* done, uint32_t* next,
+                                                            uint32_t* err) {
+  const int t = threadIdx.x & (kQueueWave - 1);
   for (;;) {
-    if (t == 0) tid_sh = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const uint32_t id = tid_sh;
-    __syncthreads();
+    const uint32_t id = wave_take(next);
     if (id >= ntasks) return;
     const HashTask tk = tasks[id];
     const BatchPolyDev& p = polys[tk.poly];
-    __shared__ int ok_sh;
-    if (t == 0) ok_sh = wait_done(done, tk.dep0, err) && wait_done(done, tk.dep1, err);
-    __syncthreads();
-    if (!ok_sh) return;
-    // the dependency flags were read with acquire by lane 0; make the data
-    // they guard visible to every lane of this wave before reading it
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!wave_wait(done, tk.dep0, err, kSpinLimit) || !wave_wait(done, tk.dep1, err, kSpinLimit))
+      return;
+    // the dependency flags were seen with relaxed loads; one acquire fence
+    // (it invalidates this XCD's L2) makes the data they guard visible to the
+    // wave.  Leaf tasks read only codewords written by earlier launches and
+    // skip it.
+    if (tk.dep0 >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (tk.kind == kTaskLeaf) {
       const uint64_t leaf = (uint64_t)tk.j * kQueueWave + t;
       if (leaf < p.leaves) {
-        const fr_t* m = p.codeword + leaf * p.leaf_elems;
-        fr_t acc = fr_load(m);
-        for (uint64_t k = 1; k < p.leaf_elems; k++) acc = sky_compress(acc, fr_load(m + k));
-        fr_store(p.nodes + leaf, acc);
+        fr_store(p.nodes + leaf,
+                          sky_hash_message(p.codeword + leaf * p.leaf_elems, p.leaf_elems));
       }
     } else if (tk.kind == kTaskNode) {
       const uint64_t nl = p.leaves >> tk.level;
@@ -102,31 +90,31 @@ __global__ __launch_bounds__(kQueueWave) void k_hash_queue(const BatchPolyDev* p
       if (node < nl) {
         const fr_t* in = p.nodes + level_off(p.leaves, tk.level - 1);
         fr_store(p.nodes + level_off(p.leaves, tk.level) + node,
-                 sky_compress(fr_load(in + 2 * node), fr_load(in + 2 * node + 1)));
+                          sky_compress(fr_load(in + 2 * node),
+                                       fr_load(in + 2 * node + 1)));
       }
-    } else {  // tail: level tk.level (<= 64 nodes) up to the root
+    } else {  // tail: level tk.level (<= 64 nodes, one per lane) up to the root
       uint64_t n = p.leaves >> tk.level;
       uint64_t off = level_off(p.leaves, tk.level);
-      if ((uint64_t)t < n) lvl[t] = fr_load(p.nodes + off + t);
-      __syncthreads();
+      fr_t cur = {};
+      if ((uint64_t)t < n) cur = fr_load(p.nodes + off + t);
       off += n;
       for (; n > 1; n >>= 1) {
         const uint64_t h = n >> 1;
-        fr_t v;
-        if ((uint64_t)t < h) v = sky_compress(lvl[2 * t], lvl[2 * t + 1]);
-        __syncthreads();
+        const fr_t a = fr_shfl(cur, (2 * t) & (kQueueWave - 1));
+        const fr_t b = fr_shfl(cur, (2 * t + 1) & (kQueueWave - 1));
         if ((uint64_t)t < h) {
-          lvl[t] = v;
-          fr_store(p.nodes + off + t, v);
+          cur = sky_compress(a, b);
+          fr_store(p.nodes + off + t, cur);
         }
-        __syncthreads();
         off += h;
       }
     }
-    // publish: every lane's stores complete before the flag
-    __threadfence();
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(done + id, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // publish: the wave's stores reach device-coherent memory (a release
+    // fence writes back this XCD's L2; no invalidate) before the flag, which
+    // every lane writes (one store of one address)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(done + id, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -287,7 +275,7 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
   PK_HIP(hipGetDevice(&dev));
   int cus = 0;
   PK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  plan->queue_blocks = (uint32_t)cus * 16;
+  plan->queue_blocks = (uint32_t)cus * 4;  // 16 waves per CU
   return plan.release();
 }
 
@@ -298,7 +286,7 @@ void batch_run(BatchPlan& plan, hipStream_t s) {
     k_coset_scale_batch<<<plan.coset_blocks, 256, 0, s>>>(plan.polys, plan.coset_map, plan.coset);
   PK_HIP(hipGetLastError());
   for (const BatchPlan::Round& rd : plan.rounds) ntt_batch_round_dev(plan.descs, rd.map, rd.blocks, rd.tl, rd.bpt, s);
-  k_hash_queue<<<plan.queue_blocks, kQueueWave, 0, s>>>(plan.polys, plan.tasks, plan.ntasks,
+  k_hash_queue<<<plan.queue_blocks, kQueueBlock, 0, s>>>(plan.polys, plan.tasks, plan.ntasks,
                                                          plan.done, plan.next, plan.err);
   k_gather_roots<<<(plan.count + 63) / 64, 64, 0, s>>>(plan.polys, plan.count, plan.roots);
   PK_HIP(hipGetLastError());

@@ -110,4 +110,66 @@ __device__ __forceinline__ fr_t sky_compress(fr_t l, fr_t r) {
   return o;
 }
 
+// SkyscraperHashEngine::hash_many of one message of `chunks` words
+// (provekit/common/src/skyscraper/whir.rs:42-100): acc = m0, then
+// acc = compress(acc, m_k).  Words are read four at a time, one 128-byte line
+// per lane: neighbouring lanes hash neighbouring messages, so reading one word
+// per compression would touch every line four times far apart in time, and
+// the lines get evicted in between.  Four compressions take tens of
+// microseconds per wave, so the other waves of the SIMD hide the next load.
+__device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks) {
+  constexpr int G = 4;
+  fr_t acc;
+  for (uint64_t g = 0; g < chunks; g += G) {
+    fr_t w[G];
+#pragma unroll
+    for (int j = 0; j < G; j++)
+      if (g + j < chunks) w[j] = fr_load(m + g + j);
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      if (g + j >= chunks) break;
+      acc = (g + j == 0) ? w[0] : sky_compress(acc, w[j]);
+    }
+  }
+  return acc;
+}
+
+// Wave-level helpers for task loops whose waves work independently (no
+// workgroup barrier).  Every lane runs them: a lane-0-only branch inside such
+// a loop lets the compiler restructure the loop so that lane 0 leaves it at
+// the end of an iteration while the other lanes go on, after which lane 0's
+// broadcasts no longer reach them.
+//
+// Next task id of the wave: one atomic (lane 0 adds 1, the others 0), lane
+// 0's result to all lanes.
+__device__ __forceinline__ uint32_t wave_take(uint32_t* counter) {
+  const uint32_t add = (threadIdx.x & 63) == 0 ? 1u : 0u;
+  const uint32_t v = __hip_atomic_fetch_add(counter, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+// Bounded wait for flag `dep` (< 0: none) with relaxed polls, each a single
+// load of one address by the whole wave with a uniform result; gives up when
+// the error flag is raised or after `spin` polls (then raises it).
+__device__ __forceinline__ bool wave_wait(const uint32_t* done, int64_t dep, uint32_t* err,
+                                          uint64_t spin) {
+  if (dep < 0) return true;
+  for (uint64_t it = 0; it < spin; it++) {
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(done + dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      return true;
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+__device__ __forceinline__ fr_t fr_shfl(const fr_t& v, int src) {
+  fr_t r;
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.l[k] = (uint32_t)__shfl((int)v.l[k], src, 64);
+  return r;
+}
+
 }  // namespace pk

@@ -22,20 +22,12 @@ __global__ __launch_bounds__(256) void k_compress_many(const fr_t* pairs, fr_t* 
   fr_store(out + i, sky_compress(l, r));
 }
 
-// One lane per message of `chunks` words: acc = m0; acc = compress(acc, m_k).
+// One lane per message of `chunks` words.
 __global__ __launch_bounds__(256) void k_hash_fold(const fr_t* in, uint64_t chunks, uint64_t count,
                                                    fr_t* out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const fr_t* m = in + i * chunks;
-  fr_t acc = fr_load(m);
-  fr_t nxt = fr_load(m + 1);
-  for (uint64_t k = 1; k < chunks; k++) {
-    const fr_t cur = nxt;
-    if (k + 1 < chunks) nxt = fr_load(m + k + 1);  // next word in flight
-    acc = sky_compress(acc, cur);
-  }
-  fr_store(out + i, acc);
+  fr_store(out + i, sky_hash_message(in + i * chunks, chunks));
 }
 
 __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out, uint64_t n_out) {

@@ -83,6 +83,20 @@ def test_commit_small(gpu, oracle, log_n, blowup, coset):
     assert root == eroot
 
 
+@pytest.mark.parametrize("log_n,blowup,leaf", [(2, 4, 16), (6, 2, 2), (8, 4, 8), (10, 2, 64), (9, 4, 1),
+                                                (11, 2, 4), (13, 2, 2), (12, 4, 32)])
+def test_commit_leaf_shapes(gpu, oracle, log_n, blowup, leaf):
+    # one leaf, tails of every size up to 64 nodes, messages that are not a
+    # multiple of the four-word read group, single-word leaves
+    c = random_fr(1 << log_n, seed=100 + log_n, stream="commit-leaves")
+    say(f"commit 2^{log_n} blowup={blowup} leaf={leaf}")
+    root, cw, nodes = gpu.commit(c, blowup, True, leaf)
+    eroot, ecw, enodes = oracle.commit(c, blowup, True, leaf)
+    assert np.array_equal(cw, ecw)
+    assert np.array_equal(nodes, enodes)
+    assert root == eroot
+
+
 def test_commit_2p20_blowup4_full_size(gpu, oracle):
     # 2^20 coefficients, blowup 4, coset NTT to 2^22, leaves of 16, Skyscraper tree
     c = random_fr(1 << 20, seed=2, stream="commit-2^20")
