@@ -99,12 +99,24 @@ def pct(xs, q):
 
 
 def load_traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(name)
-    except (OSError, ValueError):
+            return json.load(f)["kernels"].get(name)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def load_valu_busy(key):
+    """VALU busy fraction of a kernel from the committed rocprofv3 --pmc
+    summary (4 x SQ_ACTIVE_INST_VALU over SIMD cycles of the dispatch)."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[key]["busy"]
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -494,6 +506,8 @@ def main():
         "unit": "Gmodmul/s",
         "frac": achieved / MODMUL_PEAK_G,
         "traffic": load_traffic("k_hash_fold"),
+        "traffic_algorithmic": 32 * N + 32 * leaves,
+        "valu_busy_pmc": load_valu_busy("commit:k_hash_fold"),
         "work_per_launch": f"{compressions_leaf} compressions x 14 Montgomery squarings",
         "peak_basis": f"VALU int32 issue {VALU_PEAK_TOPS:.1f} T lane-instr/s / 256 instr per 8x32 Montgomery product",
     }
@@ -507,6 +521,7 @@ def main():
         "unit": "GB/s",
         "frac": ntt_bytes / (ntt_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "traffic": load_traffic("k_ntt_pass"),
+        "valu_busy_pmc": load_valu_busy("commit:k_ntt_pass<9, 1>"),
         "modmul_rate_G": ntt_modmul / (ntt_ms / 1e3) / 1e9,
         "modmul_frac": ntt_modmul / (ntt_ms / 1e3) / 1e9 / MODMUL_PEAK_G,
     }
