@@ -405,6 +405,81 @@ def workload_serving(pk, hip, s, args):
                       "witness_elements_per_s": elems / wall, "peak_in_flight": st["peak_in_flight"]})
 
 
+def workload_sumcheck(pk, hip, s, args):
+    """R1CS of 2^m constraints (1..4 nonzeros per row at uniform columns,
+    uniform Fr values, z[0] = 1): a, b, c = A z, B z, C z, then the whole
+    sumcheck over eq(r, x)(a b - c) in plain mode, all device-resident."""
+    from oracle import pyoracle
+
+    m = args.log_m
+    n = 1 << m
+    rng = np.random.default_rng(args.seed)
+    pool = pool_fr(1 << 20, args.seed, "sumcheck-pool")
+    mats_h = [csr_template(rng, n, pool) for _ in range(3)]
+    mats = [tuple(pk.DeviceBuffer.from_array(x) for x in mh) for mh in mats_h]
+    z = np.ascontiguousarray(np.tile(pool, ((n + pool.shape[0] - 1) // pool.shape[0], 1))[:n])
+    z[0] = np.array([0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E,
+                     0x0E0A77C19A07DF2F], dtype=np.uint64)  # z[0] = 1
+    dz = pk.DeviceBuffer.from_array(z)
+    abc = [pk.DeviceBuffer(32 * n) for _ in range(3)]
+    nnz = [int(mh[1].shape[0]) for mh in mats_h]
+
+    def spmv():
+        for (rs, cs, vs), k, out in zip(mats, nnz, abc):
+            pk._call("pkgpu_sparse_mul_device", n, n, rs.ptr, k, cs.ptr, vs.ptr, dz.ptr, out.ptr, s)
+
+    def prove(seed):
+        return pk.sumcheck_prove(abc[0], abc[1], abc[2], m, None, seed=seed, stream=s)
+
+    for w in range(max(1, args.warmup)):
+        spmv()
+        prove(w)
+    hip.sync(s)
+    steps = max(1, min(args.steps, 10))
+    t_spmv, t_sc = [], []
+    res = None
+    for k in range(steps):
+        e0, e1, e2 = hip.event(), hip.event(), hip.event()
+        hip.record(e0, s)
+        spmv()
+        hip.record(e1, s)
+        res = prove(1000 + k)
+        hip.record(e2, s)
+        hip.sync(s)
+        t_spmv.append(hip.elapsed_ms(e0, e1))
+        t_sc.append(hip.elapsed_ms(e1, e2))
+    spmv_ms, sc_ms = statistics.mean(t_spmv), statistics.mean(t_sc)
+    # parity of the last step at full size, and the CPU leg on the same inputs
+    t0 = time.perf_counter()
+    a_h = [pyoracle.sparse_mul(n, n, *mh, z) for mh in mats_h]
+    exp = pyoracle.sumcheck_prove(a_h[0], a_h[1], a_h[2], m, None, seed=1000 + steps - 1)
+    cpu_s = time.perf_counter() - t0
+    ok = bool(exp["ok"] and np.array_equal(res["coeffs"], exp["coeffs"]) and
+              np.array_equal(res["alphas"], exp["alphas"]) and np.array_equal(res["final"], exp["final"]))
+    spmv_bytes = sum(k * (4 + 32 + 32) for k in nnz) + 3 * n * (4 + 32)
+    sc_bytes = 512 * n  # round 0 reads a, b, c, eq; later rounds read 4 and write 4 halves
+    ms = spmv_ms + sc_ms
+    return base_line("R1CS constraints/s (Az, Bz, Cz + full sumcheck, 2^%d)" % m, n / (ms / 1e3),
+                     "constraints/s", steps, args.warmup, ms, True, "u32",
+                     {"workload": f"sumcheck: 2^{m} constraints, 1-4 nnz/row (nnz {sum(nnz)}), "
+                                  "three CSR SpMVs then the whole plain-mode sumcheck, device-resident"},
+                     {"stages_ms": {"spmv_x3": spmv_ms, "sumcheck": sc_ms},
+                      "matches_oracle": ok,
+                      "roofline_spmv": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms / 1e3) / 1e9,
+                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                        "frac": spmv_bytes / (spmv_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                        "algorithmic_bytes": spmv_bytes},
+                      "roofline_sumcheck": {"bound": "hbm", "achieved": sc_bytes / (sc_ms / 1e3) / 1e9,
+                                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                            "frac": sc_bytes / (sc_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                            "algorithmic_bytes": sc_bytes,
+                                            "note": "m rounds, each a launch of fold+map+reduce, a partial-sum "
+                                                    "launch and a one-wave transcript step"},
+                      "cpu_baseline": {"value": n / cpu_s, "unit": "constraints/s",
+                                       "cores": pyoracle.threads(), "kind": "port",
+                                       "sample": f"the same 2^{m} instance once (3 SpMVs + sumcheck) in {cpu_s:.2f} s"}})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -421,8 +496,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy kernel (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--workload", default="commit",
-                    choices=["commit", "ntt", "p2merkle", "serving"],
-                    help="commit (default: 2^20-coefficient Skyscraper commit) | ntt (forward NTT 2^16) | p2merkle (Poseidon2 tree over 2^24 leaves) | serving (Poisson R1CS jobs)")
+                    choices=["commit", "ntt", "p2merkle", "serving", "sumcheck"],
+                    help="commit (default: 2^20-coefficient Skyscraper commit) | ntt (forward NTT 2^16) | "
+                         "p2merkle (Poseidon2 tree over 2^24 leaves) | serving (Poisson R1CS jobs) | "
+                         "sumcheck (2^m R1CS SpMV + sumcheck)")
+    ap.add_argument("--log-m", type=int, default=22, help="sumcheck workload: 2^m constraints")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=60.0)
     ap.add_argument("--slots", type=int, default=8)
@@ -441,7 +519,8 @@ def main():
     hip = Hip()
     s = hip.stream()
     if args.workload != "commit":
-        fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving}[args.workload]
+        fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving,
+              "sumcheck": workload_sumcheck}[args.workload]
         print(json.dumps(fn(pk, hip, s, args)), flush=True)
         return
 
