@@ -55,6 +55,7 @@ struct Server::Job {
   fr_t* codeword = nullptr;
   fr_t* nodes = nullptr;
   bool failed = false;
+  int readers = 0;  // open() calls using the resident buffers right now
   JobResult result;
   Server* srv = nullptr;
 };
@@ -311,6 +312,9 @@ bool Server::poll(uint64_t id) {
 }
 
 void Server::open(uint64_t id, const uint64_t* leaves_idx, size_t n, fr_t* values, fr_t* paths) {
+  const fr_t* codeword;
+  const fr_t* nodes;
+  uint64_t N, leaf_elems;
   Job* j;
   {
     std::lock_guard<std::mutex> lk(mu_);
@@ -319,32 +323,60 @@ void Server::open(uint64_t id, const uint64_t* leaves_idx, size_t n, fr_t* value
     j = it->second.get();
     PK_REQUIRE(j->state == Job::Done && j->stream < 0 && j->codeword && !j->failed, PK_ERR_STATE,
                "server: job not finished, failed or released");
+    codeword = j->codeword;
+    nodes = j->nodes;
+    N = (1ull << j->spec.m) * j->spec.blowup;
+    leaf_elems = j->spec.leaf_elems;
+    j->readers++;  // release() waits until the buffers are no longer read
   }
+  // drop the reader mark on every exit path, after the stream work is done
+  struct Reader {
+    Server* s;
+    Job* j;
+    ~Reader() {
+      {
+        std::lock_guard<std::mutex> lk(s->mu_);
+        j->readers--;
+      }
+      s->done_cv_.notify_all();
+    }
+  } reader{this, j};
   if (!n) return;
-  const uint64_t N = (1ull << j->spec.m) * j->spec.blowup;
-  const uint64_t leaves = N / j->spec.leaf_elems;
+  const uint64_t leaves = N / leaf_elems;
   const int depth = log2_exact(leaves);
   for (size_t q = 0; q < n; q++)
     PK_REQUIRE(leaves_idx[q] < leaves, PK_ERR_ARG, "server: leaf index out of range");
   hipStream_t s = thread_stream();
-  Scratch di(8 * n, s), dv(32 * n * j->spec.leaf_elems, s), dp(32 * n * (depth ? depth : 1), s);
-  PK_HIP(hipMemcpyAsync(di.p, leaves_idx, 8 * n, hipMemcpyHostToDevice, s));
-  k_open<<<(unsigned)n, 64, 0, s>>>(j->codeword, j->nodes, leaves, j->spec.leaf_elems, depth,
-                                     di.as<uint64_t>(), n, dv.as<fr_t>(), dp.as<fr_t>());
-  PK_HIP(hipGetLastError());
-  PK_HIP(hipMemcpyAsync(values, dv.p, 32 * n * j->spec.leaf_elems, hipMemcpyDeviceToHost, s));
-  if (depth) PK_HIP(hipMemcpyAsync(paths, dp.p, 32 * n * depth, hipMemcpyDeviceToHost, s));
-  PK_HIP(hipStreamSynchronize(s));
+  {
+    Scratch di(8 * n, s), dv(32 * n * leaf_elems, s), dp(32 * n * (depth ? depth : 1), s);
+    PK_HIP(hipMemcpyAsync(di.p, leaves_idx, 8 * n, hipMemcpyHostToDevice, s));
+    k_open<<<(unsigned)n, 64, 0, s>>>(codeword, nodes, leaves, leaf_elems, depth,
+                                       di.as<uint64_t>(), n, dv.as<fr_t>(), dp.as<fr_t>());
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipMemcpyAsync(values, dv.p, 32 * n * leaf_elems, hipMemcpyDeviceToHost, s));
+    if (depth) PK_HIP(hipMemcpyAsync(paths, dp.p, 32 * n * depth, hipMemcpyDeviceToHost, s));
+  }
+  // also on an error above: nothing may still read the buffers when the
+  // reader mark goes
+  (void)hipStreamSynchronize(s);
 }
 
 void Server::release(uint64_t id) {
   std::unique_ptr<Job> j;
   {
-    std::lock_guard<std::mutex> lk(mu_);
+    std::unique_lock<std::mutex> lk(mu_);
     auto it = jobs_.find(id);
     PK_REQUIRE(it != jobs_.end(), PK_ERR_STATE, "server: unknown job");
     PK_REQUIRE(it->second->state == Job::Done && it->second->stream < 0, PK_ERR_STATE,
                "server: job still running");
+    // openings in progress on other threads finish first (the job is looked
+    // up again: a concurrent release may have taken it meanwhile)
+    done_cv_.wait(lk, [&] {
+      auto f = jobs_.find(id);
+      return f == jobs_.end() || f->second->readers == 0;
+    });
+    it = jobs_.find(id);
+    PK_REQUIRE(it != jobs_.end(), PK_ERR_STATE, "server: unknown job");
     j = std::move(it->second);
     jobs_.erase(it);
     if (!j->failed) in_use_ -= j->resident_bytes;

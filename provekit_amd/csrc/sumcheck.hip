@@ -35,15 +35,23 @@ __device__ __forceinline__ fr_t fr_zero() {
 }
 
 // ------------------------------------------------------------------ SpMV
-__global__ __launch_bounds__(256) void k_spmv(uint64_t rows, const uint32_t* row_starts,
-                                              uint64_t nnz, const uint32_t* cols,
-                                              const fr_t* vals, const fr_t* z, fr_t* out) {
+// Device-resident matrices are not validated on the host, so the kernel
+// keeps every access in bounds whatever the indices: row ranges are clipped
+// to [0, nnz] and entries with a column >= ncols are skipped (a malformed
+// matrix gives an unspecified result, never an out-of-bounds read).
+__global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
+                                              const uint32_t* row_starts, uint64_t nnz,
+                                              const uint32_t* cols, const fr_t* vals,
+                                              const fr_t* z, fr_t* out) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
-  const uint64_t b = row_starts[r];
-  const uint64_t e = (r + 1 < rows) ? row_starts[r + 1] : nnz;
+  const uint64_t e = (r + 1 < rows) ? min((uint64_t)row_starts[r + 1], nnz) : nnz;
+  const uint64_t b = min((uint64_t)row_starts[r], e);
   fr_t acc = fr_zero();
-  for (uint64_t i = b; i < e; i++) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + cols[i])));
+  for (uint64_t i = b; i < e; i++) {
+    const uint32_t c = cols[i];
+    if (c < ncols) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + c)));
+  }
   fr_canon(acc);
   fr_store(out + r, acc);
 }
@@ -353,7 +361,8 @@ __global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* h3,
 // ------------------------------------------------------------------ launchers
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {
   if (!A.rows) return;
-  k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.row_starts, A.nnz, A.cols, A.vals, z, out);
+  k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals,
+                                            z, out);
   PK_HIP(hipGetLastError());
 }
 

@@ -120,3 +120,41 @@ def test_budget_admission(gpu):
     assert outs[0]["root"] == outs[1]["root"] == outs[2]["root"]  # same witness
     assert not np.array_equal(outs[0]["alphas"], outs[1]["alphas"])  # different seeds
     srv.destroy()
+
+
+def test_release_waits_for_concurrent_openings(gpu):
+    import threading
+
+    import provekit_amd as pk
+    srv = gpu.Server(hbm_budget_bytes=1 << 30, slots=2)
+    m = 12
+    mats, z = make_job(m, 11)
+    jid = srv.submit(m, mats, z, seed=3)
+    srv.wait(jid, timeout_ms=60000)
+    leaves = (4 << m) // 16
+    depth = leaves.bit_length() - 1
+    ref_vals, ref_paths = srv.open(jid, [0, 5, leaves - 1], 16, depth)
+    errors, opened = [], [0]
+
+    def opener():
+        try:
+            for _ in range(200):
+                vals, paths = srv.open(jid, [0, 5, leaves - 1], 16, depth)
+                assert np.array_equal(vals, ref_vals) and np.array_equal(paths, ref_paths)
+                opened[0] += 1
+        except pk.ProvekitGpuError as e:  # released meanwhile: the only allowed failure
+            if "unknown job" not in str(e) and "released" not in str(e):
+                errors.append(e)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    th = threading.Thread(target=opener)
+    th.start()
+    while opened[0] < 5 and th.is_alive():
+        pass
+    srv.release(jid)  # waits for an opening in progress, then frees the buffers
+    th.join(timeout=60)
+    assert not th.is_alive()
+    assert not errors, errors
+    assert opened[0] >= 5
+    srv.destroy()

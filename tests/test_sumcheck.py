@@ -186,6 +186,38 @@ def test_gpu_sparse_mul(gpu, oracle, log_rows):
 
 
 @pytest.mark.gpu
+def test_gpu_sparse_mul_device_malformed_stays_in_bounds(gpu, oracle):
+    # device-resident matrices are not validated: row ranges are clipped to
+    # [0, nnz] and entries with a column >= ncols are skipped
+    rows, ncols = 64, 32
+    rng = np.random.default_rng(9)
+    cs = rng.integers(0, 48, size=200).astype(np.uint32)       # some columns >= ncols
+    rs = np.sort(rng.integers(0, 230, size=rows)).astype(np.uint32)
+    rs[5] = 250                                               # past nnz, and not monotone
+    vs = random_fr(200, seed=4, stream="bad-csr")
+    z = random_fr(ncols, seed=5, stream="bad-csr-z")
+    nnz = 200
+    # expected: the same clipping on the host
+    keep_rs, keep_cs, keep_vs = [], [], []
+    for r in range(rows):
+        e = min(int(rs[r + 1]), nnz) if r + 1 < rows else nnz
+        b = min(int(rs[r]), e)
+        keep_rs.append(len(keep_cs))
+        for i in range(b, e):
+            if cs[i] < ncols:
+                keep_cs.append(cs[i])
+                keep_vs.append(vs[i])
+    exp = oracle.sparse_mul(rows, ncols, np.array(keep_rs, np.uint32), np.array(keep_cs, np.uint32),
+                            np.array(keep_vs, np.uint64).reshape(-1, 4), z)
+    bufs = [gpu.DeviceBuffer.from_array(x) for x in (rs, cs, vs, z)]
+    out = gpu.DeviceBuffer(32 * rows)
+    gpu._call("pkgpu_sparse_mul_device", rows, ncols, bufs[0].ptr, nnz, bufs[1].ptr, bufs[2].ptr,
+              bufs[3].ptr, out.ptr, None)
+    gpu.synchronize()
+    assert np.array_equal(out.download(shape=(rows, 4)), exp)
+
+
+@pytest.mark.gpu
 def test_gpu_sparse_mul_errors_and_kats(gpu, kats):
     import provekit_amd as pk
     A = gpu.SparseMatrix(2, 3, [0, 1], [0, 2], frs([5, 6]))
