@@ -11,6 +11,9 @@
  *   squeeze(k):     out_j = C(s, [j + 1, 0, 0, 0]) for j < k, then
  *                   s = C(s, [0, 0, 0, 0]); a challenge is the Fr whose
  *                   Montgomery limbs are out_j (C reduces, so out_j < p)
+ *   round(c0..3):   absorb4(c0..3), and the new state is the round's
+ *                   challenge (a hash chain: three dependent compressions
+ *                   per round instead of four)
  * Field elements are absorbed as their 4 Montgomery limbs.
  * The ZK sumcheck driver restates run_zk_sumcheck_prover
  * (provekit/prover/src/whir_r1cs.rs:519-621) with
@@ -138,7 +141,7 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
     ofr chk = ofr_add(ofr_add(ofr_add(ofr_add(co[0], co[0]), co[1]), co[2]), co[3]);
     if (!ofr_eq(chk, S)) ok = -1;
     tr_absorb4(&tr, co);
-    tr_squeeze(&tr, 1, &alphas[idx]);
+    memcpy(alphas[idx].l, tr.s, sizeof tr.s);
     S = or_eval_cubic(co, alphas[idx]);
   }
   *final_claim = S;

@@ -298,18 +298,39 @@ __global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_
   }
 }
 
-// Round idx: h = (h(0), h(-1), h(inf)) from the reduction; form the cubic,
-// check c0 + c0 + c1 + c2 + c3 = S, absorb4, squeeze alpha_idx, S = cubic(alpha).
-__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* h3, int idx) {
+// Round idx: h = (h(0), h(-1), h(inf)) = the sum of the np workgroup
+// partials of k_sc_round; form the cubic, check c0 + c0 + c1 + c2 + c3 = S,
+// absorb4 (the new state is alpha_idx), S = cubic(alpha).
+__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* partials, unsigned np,
+                                                    int idx) {
+  __shared__ fr_t red[3][64];
   __shared__ fr_t co[4];
   __shared__ fr_t hh[2];
-  __shared__ fr_t s1;
   const int t = threadIdx.x;
+  {
+    fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero();
+    for (unsigned i = t; i < np; i += 64) {
+      s0 = fr_add(s0, fr_load(partials + 3 * i));
+      s1 = fr_add(s1, fr_load(partials + 3 * i + 1));
+      s2 = fr_add(s2, fr_load(partials + 3 * i + 2));
+    }
+    red[0][t] = s0;
+    red[1][t] = s1;
+    red[2][t] = s2;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (t < w) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) red[q][t] = fr_add(red[q][t], red[q][t + w]);
+      }
+      __syncthreads();
+    }
+  }
   if (t == 0) {
     fr_t g[4] = {fr_zero(), fr_zero(), fr_zero(), fr_zero()};
     if (st->mode == 1) blinding_coeffs(st, idx, g);
     const fr_t S = st->S, rho = st->rho;
-    const fr_t h0 = fr_load(h3), hm1 = fr_load(h3 + 1), hinf = fr_load(h3 + 2);
+    const fr_t h0 = red[0][0], hm1 = red[1][0], hinf = red[2][0];
     fr_t c0 = fr_add(h0, fr_mul(rho, g[0]));
     const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
     const fr_t at_m1 = fr_add(hm1, fr_mul(rho, gm1));
@@ -344,16 +365,12 @@ __global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* h3,
   if (t < 2) v = sky_compress(co[2 * t], co[2 * t + 1]);
   if (t < 2) hh[t] = v;
   __syncthreads();
-  if (t == 0) s1 = sky_compress(st->state, sky_compress(hh[0], hh[1]));
-  __syncthreads();
-  // squeeze(1): lane 0 the challenge, lane 1 the next state
-  if (t < 2) v = sky_compress(s1, t == 0 ? limb_word(1, 0) : fr_zero());
-  __syncthreads();
   if (t == 0) {
-    st->alpha[idx] = v;
-    st->S = cubic(st->coeffs[idx], v);
+    const fr_t s1 = sky_compress(st->state, sky_compress(hh[0], hh[1]));
+    st->alpha[idx] = s1;
+    st->state = s1;
+    st->S = cubic(st->coeffs[idx], s1);
   }
-  if (t == 1) st->state = v;
 }
 
 }  // namespace
@@ -434,9 +451,12 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
   uint64_t cur = size;
   for (int idx = 0; idx < m; idx++) {
     const fr_t* fold = idx ? &st->alpha[idx - 1] : nullptr;
-    sc_round_dev(a, b, c, eq, cur, fold, partials, h3, s);
+    // two launches per round: fold + map + workgroup sums, then the
+    // one-wave sum of the partials and transcript step
+    const unsigned nb = round_blocks(fold ? cur / 4 : cur / 2);
+    k_sc_round<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
     if (fold) cur /= 2;
-    k_sc_finalize<<<1, 64, 0, s>>>(st, h3, idx);
+    k_sc_finalize<<<1, 64, 0, s>>>(st, partials, nb, idx);
     PK_HIP(hipGetLastError());
   }
 }

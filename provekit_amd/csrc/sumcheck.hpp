@@ -6,6 +6,8 @@
 //   absorb1(x):     s = C(s, x)
 //   absorb4(x0..3): s = C(s, C(C(x0, x1), C(x2, x3)))
 //   squeeze(k):     out_j = C(s, [j + 1, 0, 0, 0]) for j < k; s = C(s, 0)
+//   round(c0..3):   absorb4(c0..3); the new state is the round challenge
+//                   (a hash chain: three dependent compressions per round)
 // Field elements enter as their Montgomery limbs; a challenge is the Fr whose
 // Montgomery limbs are out_j.  The compressions of one step that do not depend
 // on each other run on separate lanes of one wave.
