@@ -7,13 +7,14 @@
  * restates it independently of the product for the parity tests:
  *   init(seed):     s = C([seed, 0, 0, 0], [TAG0, TAG1, 0, 0])
  *   absorb1(x):     s = C(s, x)
- *   absorb4(x0..3): s = C(s, C(C(x0, x1), C(x2, x3)))
  *   squeeze(k):     out_j = C(s, [j + 1, 0, 0, 0]) for j < k, then
  *                   s = C(s, [0, 0, 0, 0]); a challenge is the Fr whose
  *                   Montgomery limbs are out_j (C reduces, so out_j < p)
- *   round(c0..3):   absorb4(c0..3), and the new state is the round's
- *                   challenge (a hash chain: three dependent compressions
- *                   per round instead of four)
+ *   round(c0..3):   s = C(C(s, c0), C(c2, c3)), and the new state is the
+ *                   round's challenge.  c1 is not absorbed: the verifier
+ *                   derives it from the claim (c1 = S - 2 c0 - c2 - c3), and
+ *                   the claim is fixed by the transcript so far.  Two
+ *                   dependent compressions per round.
  * Field elements are absorbed as their 4 Montgomery limbs.
  * The ZK sumcheck driver restates run_zk_sumcheck_prover
  * (provekit/prover/src/whir_r1cs.rs:519-621) with
@@ -36,12 +37,11 @@ static void tr_init(or_transcript* t, uint64_t seed) {
   or_sky_compress(l, r, t->s);
 }
 static void tr_absorb1(or_transcript* t, ofr x) { or_sky_compress(t->s, x.l, t->s); }
-static void tr_absorb4(or_transcript* t, const ofr x[4]) {
-  uint64_t h1[4], h2[4], h[4];
-  or_sky_compress(x[0].l, x[1].l, h1);
-  or_sky_compress(x[2].l, x[3].l, h2);
-  or_sky_compress(h1, h2, h);
-  or_sky_compress(t->s, h, t->s);
+static void tr_round(or_transcript* t, const ofr c[4]) {
+  uint64_t h1[4], h2[4];
+  or_sky_compress(t->s, c[0].l, h1);
+  or_sky_compress(c[2].l, c[3].l, h2);
+  or_sky_compress(h1, h2, t->s);
 }
 static void tr_squeeze(or_transcript* t, size_t k, ofr* out) {
   for (size_t j = 0; j < k; j++) {
@@ -140,7 +140,7 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
     co[1] = ofr_sub(ofr_sub(ofr_sub(ofr_sub(S, co[0]), co[0]), co[3]), co[2]);
     ofr chk = ofr_add(ofr_add(ofr_add(ofr_add(co[0], co[0]), co[1]), co[2]), co[3]);
     if (!ofr_eq(chk, S)) ok = -1;
-    tr_absorb4(&tr, co);
+    tr_round(&tr, co);
     memcpy(alphas[idx].l, tr.s, sizeof tr.s);
     S = or_eval_cubic(co, alphas[idx]);
   }

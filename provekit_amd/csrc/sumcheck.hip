@@ -15,7 +15,7 @@
 //    :427-492), runs the transcript step and writes the next challenge, all on
 //    one wave, so a whole sumcheck runs without the host.
 // The transcript is the project's Skyscraper transcript (init / absorb1 /
-// absorb4 / squeeze with parallel compressions, defined in sumcheck.hpp).
+// round step / squeeze with parallel compressions, defined in sumcheck.hpp).
 #include "host_fr.hpp"
 #include "kernels.hpp"
 #include "sky.hpp"
@@ -253,12 +253,13 @@ __global__ __launch_bounds__(64) void k_sc_init(ScState* st, uint64_t seed) {
   }
   __syncthreads();
   const int m = st->m;
-  // squeeze(m): r_j = C(s, [j + 1]) for lanes j < m, lane m: s = C(s, 0)
-  if (t < m) st->r[t] = sky_compress(s, limb_word(t + 1, 0));
-  fr_t ns;
-  if (t == m) ns = sky_compress(s, fr_zero());
+  // squeeze(m): r_j = C(s, [j + 1]) for lanes j < m, lane m: s = C(s, 0),
+  // one call for all lanes (operands picked per lane)
+  fr_t v;
+  if (t <= m) v = sky_compress(s, t < m ? limb_word(t + 1, 0) : fr_zero());
   __syncthreads();
-  if (t == m) st->state = ns;
+  if (t < m) st->r[t] = v;
+  if (t == m) st->state = v;
 }
 
 // After the claim reduction: mode P absorbs the computed claim; mode R
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_
 
 // Round idx: h = (h(0), h(-1), h(inf)) = the sum of the np workgroup
 // partials of k_sc_round; form the cubic, check c0 + c0 + c1 + c2 + c3 = S,
-// absorb4 (the new state is alpha_idx), S = cubic(alpha).
+// the transcript round step (the new state is alpha_idx), S = cubic(alpha).
 __global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* partials, unsigned np,
                                                     int idx) {
   __shared__ fr_t red[3][64];
@@ -360,13 +361,18 @@ __global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* par
     st->coeffs[idx][3] = c3;
   }
   __syncthreads();
-  // absorb4: lanes 0, 1 hash the coefficient pairs
-  fr_t v;
-  if (t < 2) v = sky_compress(co[2 * t], co[2 * t + 1]);
-  if (t < 2) hh[t] = v;
+  // round step: lane 0 hashes (state, c0), lane 1 (c2, c3), then one more.
+  // The operands are picked per lane before one call, so both lanes run the
+  // same instructions at once (two calls on divergent branches would run one
+  // after the other).
+  if (t < 2) {
+    const fr_t x = t == 0 ? st->state : co[2];
+    const fr_t y = t == 0 ? co[0] : co[3];
+    hh[t] = sky_compress(x, y);
+  }
   __syncthreads();
   if (t == 0) {
-    const fr_t s1 = sky_compress(st->state, sky_compress(hh[0], hh[1]));
+    const fr_t s1 = sky_compress(hh[0], hh[1]);
     st->alpha[idx] = s1;
     st->state = s1;
     st->S = cubic(st->coeffs[idx], s1);

@@ -4,10 +4,11 @@
 // C is Skyscraper compression,
 //   init(seed):     s = C([seed, 0, 0, 0], [kTrTag0, kTrTag1, 0, 0])   (u64 limbs)
 //   absorb1(x):     s = C(s, x)
-//   absorb4(x0..3): s = C(s, C(C(x0, x1), C(x2, x3)))
 //   squeeze(k):     out_j = C(s, [j + 1, 0, 0, 0]) for j < k; s = C(s, 0)
-//   round(c0..3):   absorb4(c0..3); the new state is the round challenge
-//                   (a hash chain: three dependent compressions per round)
+//   round(c0..3):   s = C(C(s, c0), C(c2, c3)); the new state is the round
+//                   challenge.  c1 is not absorbed: the verifier derives it
+//                   from the claim (c1 = S - 2 c0 - c2 - c3), which the
+//                   transcript already fixes.  Two dependent compressions.
 // Field elements enter as their Montgomery limbs; a challenge is the Fr whose
 // Montgomery limbs are out_j.  The compressions of one step that do not depend
 // on each other run on separate lanes of one wave.
