@@ -48,9 +48,26 @@ __global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
   const uint64_t e = (r + 1 < rows) ? min((uint64_t)row_starts[r + 1], nnz) : nnz;
   const uint64_t b = min((uint64_t)row_starts[r], e);
   fr_t acc = fr_zero();
-  for (uint64_t i = b; i < e; i++) {
-    const uint32_t c = cols[i];
-    if (c < ncols) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + c)));
+  // R1CS rows hold a handful of entries: take them four at a time, with all
+  // index, value and gather loads issued before the products, so a lane
+  // keeps several independent HBM requests in flight instead of one chain
+  // (index -> gather -> product) per entry
+  constexpr int G = 4;
+  for (uint64_t i0 = b; i0 < e; i0 += G) {
+    uint32_t c[G];
+    fr_t v[G], x[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) c[k] = (i0 + k < e) ? cols[i0 + k] : 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      if (c[k] < ncols) {
+        v[k] = fr_load(vals + i0 + k);
+        x[k] = fr_load(z + c[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < G; k++)
+      if (c[k] < ncols) acc = fr_add(acc, fr_mul(v[k], x[k]));
   }
   fr_canon(acc);
   fr_store(out + r, acc);
