@@ -8,67 +8,112 @@
 // permutation of [l, r, 0, 2 * 2^64] keeping lane 0
 // (playground/passport-input-gen/src/commitment.rs:225-237).
 //
-// One lane owns one permutation: 488 Montgomery products, the state in 32
-// VGPRs, round constants in constant memory read with uniform indices (they
-// arrive as scalar operands).  Values stay in [0, 2p) and leave canonical.
+// One lane owns one permutation: 488 Montgomery products, round constants
+// in constant memory read with uniform indices (scalar operands).  Inputs are
+// storage-form elements below 2^256; outputs leave canonical.
+//
+// The permutation runs in the 9 x 29-bit form of fr29.hpp: the state is
+// converted once on entry and once on exit (one product per lane each way),
+// the round constants and the internal diagonal are kept in that form, and
+// value growth is bounded as follows (units of p):
+//   - full rounds: the S-box input is below 30 p, x^2 < 6 p, x^4 < 1.2 p,
+//     x^5 < 1.3 p, so the external MDS (row sums up to 16) stays below 21 p;
+//   - partial rounds: the sum of the four lanes is reduced below 2 p, so
+//     s_i d_i + sum stays below 4 p.
+// Limbs are carried before each S-box and inside and after the MDS (whose
+// doublings would otherwise overflow 32-bit limbs).
+#include "fr29.hpp"
 #include "host_fr.hpp"
 #include "kernels.hpp"
 #include "poseidon2_params.hpp"
 
 namespace pk {
 
-__constant__ fr_t kP2RC[64][4];
-__constant__ fr_t kP2DiagM[4];
 __constant__ fr_t kP2IV2;  // 2 * 2^64, the IV of a two-element message
+__constant__ f29 kP2RC29[64][4];  // round constants, internal form
+__constant__ f29 kP2Diag29[4];
+__constant__ f29 kP2In29;         // 2^266 mod p: storage -> internal
+__constant__ f29 kP2Out29;        // 2^256 mod p: internal -> storage
 
 namespace {
 
-__device__ __forceinline__ fr_t sbox5(const fr_t& x) {
-  const fr_t x2 = fr_mul(x, x);
-  const fr_t x4 = fr_mul(x2, x2);
-  return fr_mul(x4, x);
+// x^5 in the internal form; x normalized
+__device__ __forceinline__ f29 sbox5_29(const f29& x) {
+  const f29 x2 = sqr29(x);
+  const f29 x4 = sqr29(x2);
+  return mul29(x4, x);
 }
 
-// matrix_multiplication_4x4 (permutation.rs:105-124)
-__device__ __forceinline__ void ext_mds(fr_t s[4]) {
-  const fr_t t0 = fr_add(s[0], s[1]);
-  const fr_t t1 = fr_add(s[2], s[3]);
-  const fr_t t2 = fr_add(fr_add(s[1], s[1]), t1);
-  const fr_t t3 = fr_add(fr_add(s[3], s[3]), t0);
-  fr_t t4 = fr_add(t1, t1);
-  t4 = fr_add(fr_add(t4, t4), t3);
-  fr_t t5 = fr_add(t0, t0);
-  t5 = fr_add(fr_add(t5, t5), t2);
-  const fr_t t6 = fr_add(t3, t5);
-  const fr_t t7 = fr_add(t2, t4);
-  s[0] = t6;
+// matrix_multiplication_4x4 (permutation.rs:105-124) on normalized lanes
+__device__ __forceinline__ void ext_mds29(f29 s[4]) {
+  f29 t0 = add29(s[0], s[1]);
+  f29 t1 = add29(s[2], s[3]);
+  f29 t2 = add29(add29(s[1], s[1]), t1);
+  f29 t3 = add29(add29(s[3], s[3]), t0);
+  norm29(t0);
+  norm29(t1);
+  norm29(t2);
+  norm29(t3);
+  f29 t4 = add29(t1, t1);
+  t4 = add29(add29(t4, t4), t3);
+  f29 t5 = add29(t0, t0);
+  t5 = add29(add29(t5, t5), t2);
+  s[0] = add29(t3, t5);
   s[1] = t5;
-  s[2] = t7;
+  s[2] = add29(t2, t4);
   s[3] = t4;
+  // limbs up to 6 * 2^29 here: carried so that the partial rounds' sum of
+  // four lanes and the products' operand bounds hold
+#pragma unroll
+  for (int i = 0; i < 4; i++) norm29(s[i]);
 }
 
-__device__ __forceinline__ void full_round(fr_t s[4], int r) {
+__device__ __noinline__ void p2_permute29(fr_t st[4]) {
+  f29 s[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) s[i] = sbox5(fr_add(s[i], kP2RC[r][i]));
-  ext_mds(s);
-}
-
-__device__ __noinline__ void p2_permute(fr_t s[4]) {
-  ext_mds(s);
+  for (int i = 0; i < 4; i++) s[i] = mul29(to29(st[i]), kP2In29);
+  ext_mds29(s);
 #pragma unroll 1
-  for (int r = 0; r < 4; r++) full_round(s, r);
-#pragma unroll 1
-  for (int r = 4; r < 60; r++) {
-    s[0] = sbox5(fr_add(s[0], kP2RC[r][0]));
-    const fr_t sum = fr_add(fr_add(s[0], s[1]), fr_add(s[2], s[3]));
+  for (int r = 0; r < 4; r++) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) s[i] = fr_add(fr_mul(s[i], kP2DiagM[i]), sum);
+    for (int i = 0; i < 4; i++) {
+      f29 x = add29(s[i], kP2RC29[r][i]);
+      norm29(x);
+      s[i] = sbox5_29(x);
+    }
+    ext_mds29(s);
   }
 #pragma unroll 1
-  for (int r = 60; r < 64; r++) full_round(s, r);
+  for (int r = 4; r < 60; r++) {
+    f29 x = add29(s[0], kP2RC29[r][0]);
+    norm29(x);
+    s[0] = sbox5_29(x);
+    f29 sum = add29(add29(s[0], s[1]), add29(s[2], s[3]));
+    norm29(sum);
+    reduce29(sum);
 #pragma unroll
-  for (int i = 0; i < 4; i++) fr_canon(s[i]);
+    for (int i = 0; i < 4; i++) s[i] = add29(mul29(s[i], kP2Diag29[i]), sum);
+  }
+#pragma unroll 1
+  for (int r = 60; r < 64; r++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      f29 x = add29(s[i], kP2RC29[r][i]);
+      norm29(x);
+      s[i] = sbox5_29(x);
+    }
+    ext_mds29(s);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    f29 v = mul29(s[i], kP2Out29);  // < 1.2 p: one subtraction makes it canonical
+    norm29(v);
+    st[i] = from29(v);
+    fr_canon(st[i]);
+  }
 }
+
+__device__ __forceinline__ void p2_permute(fr_t s[4]) { p2_permute29(s); }
 
 __device__ __forceinline__ fr_t p2_two_to_one(const fr_t& l, const fr_t& r) {
   fr_t s[4];
@@ -152,7 +197,9 @@ std::mutex g_p2_mtx;
 
 }  // namespace
 
-// Parameters in Montgomery form in this device's constant memory (once).
+// Parameters in this device's constant memory (once): the round constants
+// and the internal diagonal in the 29-bit internal form, the IV of a
+// two-element message in storage form.
 void poseidon2_consts(Device& d) {
   std::lock_guard<std::mutex> lk(g_p2_mtx);
   if (!d.p2_ready) {
@@ -168,9 +215,35 @@ void poseidon2_consts(Device& d) {
     for (int i = 0; i < 4; i++) put(diag[i], hf::from_hex(kP2Diag[i]));
     fr_t iv2;
     put(iv2, hf::mul(hf::from_u64(2), hf::from_int(hfr{{0, 1, 0, 0}})));
-    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2RC), rc, sizeof rc));
-    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2DiagM), diag, sizeof diag));
     PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2IV2), &iv2, sizeof iv2));
+    // internal (2^261 Montgomery, 29-bit limbs) forms: I(c) = S(c) * 32 mod p
+    const hfr s32 = hf::from_u64(32);
+    auto limbs29 = [](f29& dst, const hfr& v) {  // v: an integer below 2^256
+      for (int k = 0; k < 9; k++) {
+        const int b = 29 * k, w = b >> 6, sh = b & 63;
+        unsigned __int128 x = v.l[w] >> sh;
+        if (w + 1 < 4 && sh > 35) x |= (unsigned __int128)v.l[w + 1] << (64 - sh);
+        dst.l[k] = (uint32_t)(x & 0x1fffffffu);
+      }
+    };
+    static f29 rc29[64][4], diag29[4], in29, out29;
+    for (int r = 0; r < 64; r++)
+      for (int j = 0; j < 4; j++) {
+        hfr v;
+        memcpy(&v, &rc[r][j], 32);
+        limbs29(rc29[r][j], hf::mul(v, s32));
+      }
+    for (int i = 0; i < 4; i++) {
+      hfr v;
+      memcpy(&v, &diag[i], 32);
+      limbs29(diag29[i], hf::mul(v, s32));
+    }
+    limbs29(in29, hf::from_u64(1024));  // 1024 * 2^256 = 2^266 (mod p)
+    limbs29(out29, hf::ONE);            // 2^256 (mod p)
+    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2RC29), rc29, sizeof rc29));
+    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2Diag29), diag29, sizeof diag29));
+    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2In29), &in29, sizeof in29));
+    PK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kP2Out29), &out29, sizeof out29));
     d.p2_ready = true;
   }
 }
