@@ -405,6 +405,52 @@ def workload_serving(pk, hip, s, args):
                       "witness_elements_per_s": elems / wall, "peak_in_flight": st["peak_in_flight"]})
 
 
+def workload_pow(pk, hip, s, args):
+    """Skyscraper proof-of-work grinding: for each of `steps` seeded challenges,
+    the smallest nonce whose compression is below threshold(bits + 0.01)."""
+    from oracle import pyoracle
+
+    bits = args.pow_bits
+    chal = pool_fr(max(1, args.steps) + 2, args.seed, "pow-challenges")
+    pk.pow.solve(chal[-1], 10.0, stream=s)  # warm-up: kernel load, first windows
+    hip.sync(s)
+    steps = max(1, min(args.steps, 5))
+    nonces, gpu_ms = [], []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, s)
+        nonces.append(pk.pow.solve(chal[k], bits, stream=s))
+        hip.record(e1, s)
+        hip.sync(s)
+        gpu_ms.append(hip.elapsed_ms(e0, e1))
+    wall = time.perf_counter() - t0
+    searched = sum(n + 1 for n in nonces)
+    rate = searched / (sum(gpu_ms) / 1e3)
+    # the answers pass, and the nonce below each fails (full minimality is
+    # checked against the oracle's scan in the tests, at up to 19 bits)
+    ok = all(pyoracle.pow_verify(chal[k], bits + 0.01, n) and
+             (n == 0 or not pyoracle.pow_verify(chal[k], bits + 0.01, n - 1))
+             for k, n in enumerate(nonces))
+    cpu_bits = min(bits, 25.0)
+    t1 = time.perf_counter()
+    cpu_n = pyoracle.pow_solve(chal[-2], cpu_bits)
+    cpu_s = time.perf_counter() - t1
+    gm = rate * 14 / 1e9
+    return base_line(f"nonces searched/s (Skyscraper PoW, {bits:g} bits)", rate, "nonces/s", steps, 1,
+                     sum(gpu_ms) / steps, True, "u32",
+                     {"workload": f"proof-of-work grinding at {bits:g} bits (threshold at {bits + 0.01:g}), "
+                                  f"{steps} seeded challenges, smallest passing nonce each"},
+                     {"nonces": nonces, "wall_s": wall, "answers_pass_oracle": bool(ok),
+                      "roofline": {"bound": "valu", "kernel": "k_pow_window (one compression per lane)",
+                                   "achieved": gm, "peak": MODMUL_PEAK_G, "unit": "Gmodmul/s",
+                                   "frac": gm / MODMUL_PEAK_G,
+                                   "work_per_launch": "one compression (14 Montgomery squarings) per nonce"},
+                      "cpu_baseline": {"value": (cpu_n + 1) / cpu_s, "unit": "nonces/s",
+                                       "cores": pyoracle.threads(), "kind": "port",
+                                       "sample": f"one solve at {cpu_bits:g} bits: nonce {cpu_n} in {cpu_s:.2f} s"}})
+
+
 def workload_sumcheck(pk, hip, s, args):
     """R1CS of 2^m constraints (1..4 nonzeros per row at uniform columns,
     uniform Fr values, z[0] = 1): a, b, c = A z, B z, C z, then the whole
@@ -496,10 +542,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy kernel (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--workload", default="commit",
-                    choices=["commit", "ntt", "p2merkle", "serving", "sumcheck"],
+                    choices=["commit", "ntt", "p2merkle", "serving", "sumcheck", "pow"],
                     help="commit (default: 2^20-coefficient Skyscraper commit) | ntt (forward NTT 2^16) | "
                          "p2merkle (Poseidon2 tree over 2^24 leaves) | serving (Poisson R1CS jobs) | "
-                         "sumcheck (2^m R1CS SpMV + sumcheck)")
+                         "sumcheck (2^m R1CS SpMV + sumcheck) | pow (Skyscraper proof-of-work grinding)")
+    ap.add_argument("--pow-bits", type=float, default=30.0, help="pow workload: difficulty in bits")
     ap.add_argument("--log-m", type=int, default=22, help="sumcheck workload: 2^m constraints")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=60.0)
@@ -520,7 +567,7 @@ def main():
     s = hip.stream()
     if args.workload != "commit":
         fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving,
-              "sumcheck": workload_sumcheck}[args.workload]
+              "sumcheck": workload_sumcheck, "pow": workload_pow}[args.workload]
         print(json.dumps(fn(pk, hip, s, args)), flush=True)
         return
 

@@ -132,6 +132,25 @@ int pkgpu_skyscraper_hash_many_device(size_t size, const void* input, size_t cou
  * last (nodes may equal leaves). */
 int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, void* stream);
 
+/* ------------------------------------------------------ proof of work */
+/* skyscraper::pow::threshold (skyscraper/core/src/pow.rs:18-26): 4 LE u64.
+ * PKGPU_ERR_ARG ("Difficulty must be in the range [0, 80)") outside [0, 80),
+ * where the reference panics. */
+int pkgpu_skyscraper_pow_threshold(double difficulty, uint64_t out[4]);
+/* skyscraper::pow::verify (pow.rs:28-30): *ok = 1 when difficulty is 0 or
+ * compress(challenge, [nonce, 0, 0, 0]) < threshold(difficulty).  The
+ * challenge is any 256-bit integer (4 LE u64, the reference's
+ * transmute of the 32-byte spongefish challenge). */
+int pkgpu_skyscraper_pow_verify(const uint64_t challenge[4], double difficulty, uint64_t nonce,
+                                int* ok);
+/* skyscraper::pow::solve (pow.rs:36-45, generic.rs:42-71): the smallest nonce
+ * passing threshold(difficulty + 0.01); 0 when difficulty is 0.  Searches
+ * nonces [0, max_nonces) (max_nonces 0: no bound, as the reference) and sets
+ * *found = 0 if none passes.  Synchronises `stream` (NULL: the calling
+ * thread's stream) once per window of nonces. */
+int pkgpu_skyscraper_pow_solve(const uint64_t challenge[4], double difficulty,
+                               uint64_t max_nonces, uint64_t* nonce, int* found, void* stream);
+
 /* ------------------------------------------------------------ Poseidon2 */
 /* poseidon2::poseidon2_permutation (poseidon2/src/permutation.rs:174-176) on
  * n states of 4 Fr (4n elements, row-major). */

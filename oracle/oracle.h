@@ -44,6 +44,11 @@ void or_sky_compress(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]);
 void or_scalar_sqr(const uint64_t a[4], uint64_t out[4]);
 void or_sky_compress_many(const uint8_t* messages, uint8_t* hashes, size_t count);
 int or_sky_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output);
+/* proof of work (skyscraper/core/src/pow.rs) */
+void or_f64_to_u256(double f, uint64_t out[4]);
+int or_pow_threshold(double difficulty, uint64_t out[4]);
+int or_pow_verify(const uint64_t challenge[4], double difficulty, uint64_t nonce);
+int or_pow_solve(const uint64_t challenge[4], double difficulty, uint64_t limit, uint64_t* nonce);
 
 /* ---- commit: RS encode + leaf hashing + binary Skyscraper tree ---- */
 /* tree: 2*num_leaves-1 digests: nodes[0..num_leaves) leaves, then each level, root last */

@@ -53,6 +53,10 @@ def lib():
             "or_scalar_sqr": ([_vp, _vp], None),
             "or_sky_compress_many": ([_vp, _vp, _sz], None),
             "or_sky_hash_many": ([_sz, _vp, _sz, _vp], _i),
+            "or_f64_to_u256": ([ctypes.c_double, _vp], None),
+            "or_pow_threshold": ([ctypes.c_double, _vp], _i),
+            "or_pow_verify": ([_vp, ctypes.c_double, _u64], _i),
+            "or_pow_solve": ([_vp, ctypes.c_double, _u64, ctypes.POINTER(_u64)], _i),
             "or_merkle_tree": ([_vp, _sz, _vp], None),
             "or_commit": ([_vp, _sz, _sz, _i, _sz, _vp, _vp], _i),
             "or_p2_permutation": ([_vp, _vp], None),
@@ -228,6 +232,34 @@ def sky_hash_many(size: int, data: bytes, count: int) -> bytes:
     if lib().or_sky_hash_many(size, _p(m), count, _p(out)):
         raise ValueError("unsupported size")
     return out.tobytes()
+
+
+def f64_to_u256(f: float) -> np.ndarray:
+    o = np.zeros(4, np.uint64)
+    lib().or_f64_to_u256(float(f), _p(o))
+    return o
+
+
+def pow_threshold(difficulty: float) -> np.ndarray:
+    o = np.zeros(4, np.uint64)
+    if lib().or_pow_threshold(float(difficulty), _p(o)):
+        raise ValueError("Difficulty must be in the range [0, 80)")
+    return o
+
+
+def pow_verify(challenge, difficulty: float, nonce: int) -> bool:
+    r = lib().or_pow_verify(_p(_a(challenge)), float(difficulty), int(nonce))
+    if r < 0:
+        raise ValueError("Difficulty must be in the range [0, 80)")
+    return bool(r)
+
+
+def pow_solve(challenge, difficulty: float, limit: int = 1 << 40) -> Optional[int]:
+    n = _u64(0)
+    r = lib().or_pow_solve(_p(_a(challenge)), float(difficulty), int(limit), ctypes.byref(n))
+    if r < 0:
+        raise ValueError("Difficulty must be in the range [0, 80)")
+    return int(n.value) if r else None
 
 
 def merkle_tree(leaf_digests: np.ndarray) -> np.ndarray:

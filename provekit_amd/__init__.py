@@ -65,6 +65,10 @@ SIGNATURES = {
     "pkgpu_skyscraper_hash_many": [_sz, _vp, _sz, _vp, _sz],
     "pkgpu_skyscraper_hash_many_device": [_sz, _vp, _sz, _vp, _vp],
     "pkgpu_skyscraper_merkle_device": [_vp, _sz, _vp, _vp],
+    "pkgpu_skyscraper_pow_threshold": [ctypes.c_double, _vp],
+    "pkgpu_skyscraper_pow_verify": [_vp, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_i)],
+    "pkgpu_skyscraper_pow_solve": [_vp, ctypes.c_double, ctypes.c_uint64,
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_i), _vp],
     "pkgpu_poseidon2_permutation_many": [_vp, _sz, _vp],
     "pkgpu_poseidon2_hash": [_vp, _sz, _vp],
     "pkgpu_poseidon2_hash_many": [_vp, _sz, _sz, _vp],
@@ -264,6 +268,61 @@ class skyscraper:  # noqa: N801 - namespace mirroring the `skyscraper` crate
         _call("pkgpu_skyscraper_compress_many", _ptr(m) if len(m) else None, len(m),
               _ptr(out) if len(out) else None, len(out))
         return out.tobytes()
+
+
+class pow:  # noqa: N801 - namespace mirroring `skyscraper::pow`
+    """skyscraper::pow (skyscraper/core/src/pow.rs:18-45).  A challenge is any
+    256-bit integer as 4 LE u64."""
+
+    @staticmethod
+    def threshold(difficulty: float) -> np.ndarray:
+        out = np.zeros(4, dtype=np.uint64)
+        _call("pkgpu_skyscraper_pow_threshold", float(difficulty), _ptr(out))
+        return out
+
+    @staticmethod
+    def verify(challenge, difficulty: float, nonce: int) -> bool:
+        _ensure_init()
+        c = np.ascontiguousarray(challenge, dtype=np.uint64).reshape(4)
+        ok = _i(0)
+        _call("pkgpu_skyscraper_pow_verify", _ptr(c), float(difficulty), int(nonce),
+              ctypes.byref(ok))
+        return bool(ok.value)
+
+    @staticmethod
+    def solve(challenge, difficulty: float, max_nonces: int = 0, stream=None) -> Optional[int]:
+        """The smallest nonce passing threshold(difficulty + 0.01); None if
+        none below max_nonces (0: unbounded, as the reference)."""
+        _ensure_init()
+        c = np.ascontiguousarray(challenge, dtype=np.uint64).reshape(4)
+        n, f = ctypes.c_uint64(0), _i(0)
+        _call("pkgpu_skyscraper_pow_solve", _ptr(c), float(difficulty), int(max_nonces),
+              ctypes.byref(n), ctypes.byref(f), stream)
+        return int(n.value) if f.value else None
+
+
+class SkyscraperPoW:
+    """spongefish PowStrategy (provekit/common/src/skyscraper/pow.rs:8-36)."""
+
+    def __init__(self, challenge: bytes, bits: float):
+        if not (0.0 <= bits < 60.0):
+            raise ValueError("bits must be smaller than 60")
+        if len(challenge) != 32:
+            raise ValueError("challenge must be 32 bytes")
+        self.challenge = bytes(challenge)
+        self.bits = float(bits)
+
+    def _c(self) -> np.ndarray:
+        return np.frombuffer(self.challenge, dtype=np.uint64)
+
+    def check(self, nonce: int) -> bool:
+        return pow.verify(self._c(), self.bits, nonce)
+
+    def solution(self, nonce: int) -> dict:
+        return {"challenge": self.challenge, "nonce": int(nonce)}
+
+    def solve(self) -> dict:
+        return self.solution(pow.solve(self._c(), self.bits))
 
 
 class SkyscraperHashEngine:

@@ -24,6 +24,7 @@ SOURCES = [
     "api.cpp",
     "ntt.hip",
     "skyscraper.hip",
+    "pow.hip",
     "commit.hip",
     "poseidon2.hip",
     "sumcheck.hip",

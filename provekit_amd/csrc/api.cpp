@@ -177,7 +177,45 @@ Fr compress(const Fr& l, const Fr& r) {
   compress_many(m, 64, reinterpret_cast<uint8_t*>(o.data()), 32);
   return o;
 }
+
+namespace pow {
+Fr threshold(double difficulty) {
+  Fr t;
+  pk::pow_threshold(difficulty, t.data());
+  return t;
+}
+bool verify(const Fr& challenge, double difficulty, uint64_t nonce) {
+  if (difficulty == 0.0) return true;
+  const Fr t = threshold(difficulty);
+  const Fr h = compress(challenge, Fr{nonce, 0, 0, 0});
+  for (int i = 3; i >= 0; i--)
+    if (h[i] != t[i]) return h[i] < t[i];
+  return false;
+}
+uint64_t solve(const Fr& challenge, double difficulty) {
+  uint64_t nonce = 0;
+  const bool found =
+      pk::sky_pow_solve_dev(challenge.data(), difficulty, ~0ULL, &nonce, pk::thread_stream());
+  REQ(found, pk::PK_ERR_INTERNAL, "proof of work: no nonce below 2^64 - 1");
+  return nonce;
+}
+}  // namespace pow
 }  // namespace skyscraper
+
+SkyscraperPoW::SkyscraperPoW(const std::array<uint8_t, 32>& challenge, double bits)
+    : challenge_(challenge), bits_(bits) {
+  REQ(bits >= 0.0 && bits < 60.0, pk::PK_ERR_ARG, "bits must be smaller than 60");
+}
+bool SkyscraperPoW::check(uint64_t nonce) const {
+  Fr c;
+  memcpy(c.data(), challenge_.data(), 32);
+  return skyscraper::pow::verify(c, bits_, nonce);
+}
+SkyscraperPoW::Solution SkyscraperPoW::solve() const {
+  Fr c;
+  memcpy(c.data(), challenge_.data(), 32);
+  return Solution{challenge_, skyscraper::pow::solve(c, bits_)};
+}
 
 void SkyscraperHashEngine::hash_many(size_t size, const uint8_t* input, size_t input_len,
                                      Hash* output, size_t count) const {
@@ -444,6 +482,32 @@ int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, vo
     REQ(leaves && nodes, pk::PK_ERR_ARG, "merkle: null pointer");
     pk::sky_merkle_dev(static_cast<const fr_t*>(leaves), n, static_cast<fr_t*>(nodes),
                        stream_or_thread(stream));
+  });
+}
+
+int pkgpu_skyscraper_pow_threshold(double difficulty, uint64_t out[4]) {
+  return guarded([&] {
+    REQ(out, pk::PK_ERR_ARG, "pow_threshold: null pointer");
+    pk::pow_threshold(difficulty, out);
+  });
+}
+int pkgpu_skyscraper_pow_verify(const uint64_t challenge[4], double difficulty, uint64_t nonce,
+                                int* ok) {
+  return guarded([&] {
+    REQ(challenge && ok, pk::PK_ERR_ARG, "pow_verify: null pointer");
+    Fr c;
+    memcpy(c.data(), challenge, 32);
+    *ok = skyscraper::pow::verify(c, difficulty, nonce) ? 1 : 0;
+  });
+}
+int pkgpu_skyscraper_pow_solve(const uint64_t challenge[4], double difficulty,
+                               uint64_t max_nonces, uint64_t* nonce, int* found, void* stream) {
+  return guarded([&] {
+    REQ(challenge && nonce && found, pk::PK_ERR_ARG, "pow_solve: null pointer");
+    *found = pk::sky_pow_solve_dev(challenge, difficulty, max_nonces ? max_nonces : ~0ULL, nonce,
+                                   stream_or_thread(stream))
+                 ? 1
+                 : 0;
   });
 }
 

@@ -74,6 +74,17 @@ void sky_hash_many_dev(uint64_t chunks, const fr_t* in, uint64_t count, fr_t* ou
 // alias `leaves`), levels follow, root at nodes[2n-2].
 void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s);
 
+// ---- proof of work (pow.hip) ----
+// f64_to_u256 and threshold of skyscraper/core/src/pow.rs:18-26,61-88
+// (threshold throws PK_ERR_ARG outside [0, 80)).
+void f64_to_u256(double f, uint64_t out[4]);
+void pow_threshold(double difficulty, uint64_t out[4]);
+// The smallest nonce below `limit` whose compression with `challenge` (any
+// 256-bit integer) is below threshold(difficulty + 0.01); false if none.
+// Synchronises `s` once per window of nonces.
+bool sky_pow_solve_dev(const uint64_t challenge[4], double difficulty, uint64_t limit,
+                       uint64_t* nonce, hipStream_t s);
+
 // ---- Poseidon2 (poseidon2.hip) ----
 // n states of 4 elements (row-major) -> permuted states.
 void p2_permutation_many_dev(Device& d, const fr_t* states, fr_t* out, uint64_t n, hipStream_t s);

@@ -6,6 +6,8 @@
 //   RSFr : ReedSolomon<Fr>                      provekit/common/src/ntt.rs:12-105
 //   skyscraper::simple::{compress, compress_many} skyscraper/core/src/simple.rs:3-14
 //   SkyscraperHashEngine : HashEngine           provekit/common/src/skyscraper/whir.rs:29-101
+//   skyscraper::pow, SkyscraperPoW              skyscraper/core/src/pow.rs:18-45,
+//                                               provekit/common/src/skyscraper/pow.rs:8-36
 // Host slices are copied to HBM, transformed there, and copied back.
 #pragma once
 #include <array>
@@ -37,7 +39,30 @@ namespace skyscraper {
 Fr compress(const Fr& l, const Fr& r);
 void compress_many(const uint8_t* messages, size_t messages_len, uint8_t* hashes,
                    size_t hashes_len);
+// skyscraper::pow (skyscraper/core/src/pow.rs:18-45).  The challenge is any
+// 256-bit integer (4 LE u64).
+namespace pow {
+Fr threshold(double difficulty);
+bool verify(const Fr& challenge, double difficulty, uint64_t nonce);
+uint64_t solve(const Fr& challenge, double difficulty);
+}  // namespace pow
 }  // namespace skyscraper
+
+// SkyscraperPoW : spongefish PowStrategy (provekit/common/src/skyscraper/pow.rs:8-36)
+struct SkyscraperPoW {
+  struct Solution {
+    std::array<uint8_t, 32> challenge;
+    uint64_t nonce;
+  };
+  SkyscraperPoW(const std::array<uint8_t, 32>& challenge, double bits);
+  bool check(uint64_t nonce) const;
+  Solution solution(uint64_t nonce) const { return Solution{challenge_, nonce}; }
+  Solution solve() const;
+
+ private:
+  std::array<uint8_t, 32> challenge_;
+  double bits_;
+};
 
 struct SkyscraperHashEngine {
   std::string name() const { return "skyscraper"; }
