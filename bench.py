@@ -495,6 +495,42 @@ def workload_sumcheck(pk, hip, s, args):
         t_spmv.append(hip.elapsed_ms(e0, e1))
         t_sc.append(hip.elapsed_ms(e1, e2))
     spmv_ms, sc_ms = statistics.mean(t_spmv), statistics.mean(t_sc)
+    # after the sumcheck (whir_r1cs.rs:172-173): transpose A, B, C and
+    # multiply each by eq(alpha, .); alpha = the last prove's challenges
+    tmats = [(pk.DeviceBuffer(4 * n), pk.DeviceBuffer(4 * k), pk.DeviceBuffer(32 * k)) for k in nnz]
+    dalpha = pk.DeviceBuffer.from_array(np.ascontiguousarray(res["alphas"], dtype=np.uint64))
+    outs = [pk.DeviceBuffer(32 * n) for _ in range(3)]
+
+    def transpose_all():
+        for (rs, cs, vs), k, (trs, tcs, tvs) in zip(mats, nnz, tmats):
+            pk._call("pkgpu_sparse_transpose_device", n, n, rs.ptr, k, cs.ptr, vs.ptr,
+                     trs.ptr, tcs.ptr, tvs.ptr, s)
+
+    def at_eq_all():
+        for (trs, tcs, tvs), k, out in zip(tmats, nnz, outs):
+            pk._call("pkgpu_multiply_transposed_by_eq_alpha_device", n, n, trs.ptr, k, tcs.ptr,
+                     tvs.ptr, dalpha.ptr, m, out.ptr, s)
+
+    transpose_all()
+    at_eq_all()
+    hip.sync(s)
+    t_tr, t_eq = [], []
+    for _ in range(3):
+        e0, e1, e2 = hip.event(), hip.event(), hip.event()
+        hip.record(e0, s)
+        transpose_all()
+        hip.record(e1, s)
+        at_eq_all()
+        hip.record(e2, s)
+        hip.sync(s)
+        t_tr.append(hip.elapsed_ms(e0, e1))
+        t_eq.append(hip.elapsed_ms(e1, e2))
+    tr_ms, eq_ms = statistics.mean(t_tr), statistics.mean(t_eq)
+    # parity of the last products against the oracle (transpose + eq + SpMV)
+    eq_h = pyoracle.eq_evals(res["alphas"], n)
+    at_ok = all(np.array_equal(out.download(shape=(n, 4)),
+                               pyoracle.sparse_mul(n, n, *pyoracle.sparse_transpose(n, n, *mh), eq_h))
+                for out, mh in zip(outs, mats_h))
     # parity of the last step at full size, and the CPU leg on the same inputs
     t0 = time.perf_counter()
     a_h = [pyoracle.sparse_mul(n, n, *mh, z) for mh in mats_h]
@@ -511,6 +547,13 @@ def workload_sumcheck(pk, hip, s, args):
                                   "three CSR SpMVs then the whole plain-mode sumcheck, device-resident"},
                      {"stages_ms": {"spmv_x3": spmv_ms, "sumcheck": sc_ms},
                       "matches_oracle": ok,
+                      "after_sumcheck": {
+                          "transpose_x3_ms": tr_ms, "at_eq_alpha_x3_ms": eq_ms,
+                          "matches_oracle": bool(at_ok),
+                          "transpose_hbm_frac": sum(72 * k + 8 * n for k in nnz) / (tr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "at_eq_hbm_frac": (spmv_bytes + 3 * 32 * n) / (eq_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "note": "not in value: SparseMatrix::transpose of A, B, C and At/Bt/Ct * eq(alpha) "
+                                  "(whir_r1cs.rs:172-173); algorithmic bytes 72 nnz + 8 n per transpose"},
                       "roofline_spmv": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms / 1e3) / 1e9,
                                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                         "frac": spmv_bytes / (spmv_ms / 1e3) / 1e9 / HBM_PEAK_GBS,

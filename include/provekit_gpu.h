@@ -186,6 +186,29 @@ int pkgpu_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_start
 int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
                             size_t nnz, const uint32_t* cols, const uint64_t* vals,
                             const uint64_t* z, uint64_t* out, void* stream);
+/* SparseMatrix::transpose (provekit/common/src/sparse_matrix.rs:409-449):
+ * the num_cols x num_rows matrix whose entries are the (col, row, value)
+ * triples in (col, row) order.  t_row_starts receives num_cols entries (row c
+ * ends where row c+1 starts, the last at nnz), t_cols and t_vals nnz each.
+ * The host-slice form checks the input like pkgpu_sparse_mul; the device form
+ * trusts it (column indices below num_cols) and stays in bounds regardless. */
+int pkgpu_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                           size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                           uint32_t* t_row_starts, uint32_t* t_cols, uint64_t* t_vals);
+int pkgpu_sparse_transpose_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                  size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                                  uint32_t* t_row_starts, uint32_t* t_cols, uint64_t* t_vals,
+                                  void* stream);
+/* One matrix of multiply_transposed_by_eq_alpha
+ * (provekit/common/src/utils/sumcheck.rs:203-227): out (t_num_rows Fr) =
+ * At * eq(alpha, .), the eq table over the m device-resident elements of
+ * alpha truncated to t_num_cols entries (the constraint count).  At is the
+ * transposed matrix, e.g. from pkgpu_sparse_transpose_device. */
+int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num_cols,
+                                                 const uint32_t* t_row_starts, size_t nnz,
+                                                 const uint32_t* t_cols, const uint64_t* t_vals,
+                                                 const uint64_t* alpha, size_t m, uint64_t* out,
+                                                 void* stream);
 /* calculate_witness_bounds (provekit/common/src/utils/sumcheck.rs:163-179):
  * a = A w, b = B w, c = a o b, each zero-padded to padded_len =
  * num_rows.next_power_of_two() (written to *padded_len; the out arrays must

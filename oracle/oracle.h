@@ -66,6 +66,9 @@ void or_eq_evals(const ofr* r, size_t m, size_t num_entries, ofr* out);
 ofr or_calculate_eq(const ofr* r, const ofr* alpha, size_t n);
 void or_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
                    const uint32_t* cols, const ofr* vals, const ofr* z, ofr* out);
+int or_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
+                        const uint32_t* cols, const ofr* vals, uint32_t* t_row_starts,
+                        uint32_t* t_cols, ofr* t_vals);
 void or_sumcheck_r1cs_round(ofr* a, ofr* b, ofr* c, ofr* eq, size_t size, const ofr* fold,
                             ofr out[3]);
 ofr or_eval_cubic(const ofr poly[4], ofr x);

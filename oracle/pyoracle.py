@@ -65,6 +65,7 @@ def lib():
             "or_eq_evals": ([_vp, _sz, _sz, _vp], None),
             "or_calculate_eq": ([_vp, _vp, _sz], OFr),
             "or_sparse_mul": ([_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp], None),
+            "or_sparse_transpose": ([_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp], _i),
             "or_sumcheck_r1cs_round": ([_vp, _vp, _vp, _vp, _sz, _vp, _vp], None),
             "or_eval_cubic": ([_vp, OFr], OFr),
             "or_sumcheck_prove": ([_vp, _vp, _vp, _sz, _vp, _u64, _vp, _vp, _vp, _vp], _i),
@@ -248,7 +249,8 @@ def pow_threshold(difficulty: float) -> np.ndarray:
 
 
 def pow_verify(challenge, difficulty: float, nonce: int) -> bool:
-    r = lib().or_pow_verify(_p(_a(challenge)), float(difficulty), int(nonce))
+    ch = _a(challenge)
+    r = lib().or_pow_verify(_p(ch), float(difficulty), int(nonce))
     if r < 0:
         raise ValueError("Difficulty must be in the range [0, 80)")
     return bool(r)
@@ -256,7 +258,8 @@ def pow_verify(challenge, difficulty: float, nonce: int) -> bool:
 
 def pow_solve(challenge, difficulty: float, limit: int = 1 << 40) -> Optional[int]:
     n = _u64(0)
-    r = lib().or_pow_solve(_p(_a(challenge)), float(difficulty), int(limit), ctypes.byref(n))
+    ch = _a(challenge)
+    r = lib().or_pow_solve(_p(ch), float(difficulty), int(limit), ctypes.byref(n))
     if r < 0:
         raise ValueError("Difficulty must be in the range [0, 80)")
     return int(n.value) if r else None
@@ -325,6 +328,20 @@ def sparse_mul(num_rows: int, num_cols: int, row_starts, cols, vals, z) -> np.nd
     out = np.zeros((num_rows, 4), np.uint64)
     lib().or_sparse_mul(num_rows, num_cols, _p(rs), cs.shape[0], _p(cs), _p(vs), _p(zz), _p(out))
     return out
+
+
+def sparse_transpose(num_rows: int, num_cols: int, row_starts, cols, vals):
+    """(t_row_starts, t_cols, t_vals) of the transposed matrix (num_cols rows)."""
+    rs = np.ascontiguousarray(row_starts, dtype=np.uint32)
+    cs = np.ascontiguousarray(cols, dtype=np.uint32)
+    vs = np.ascontiguousarray(vals, dtype=np.uint64).reshape(-1, 4)
+    nnz = cs.shape[0]
+    trs = np.zeros(num_cols, np.uint32)
+    tcs = np.zeros(nnz, np.uint32)
+    tvs = np.zeros((nnz, 4), np.uint64)
+    if lib().or_sparse_transpose(num_rows, num_cols, _p(rs), nnz, _p(cs), _p(vs), _p(trs), _p(tcs), _p(tvs)):
+        raise ValueError("column index out of range")
+    return trs, tcs, tvs
 
 
 def sumcheck_round(a, b, c, eq, fold=None) -> np.ndarray:

@@ -58,6 +58,37 @@ void or_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
   }
 }
 
+/* SparseMatrix::transpose (provekit/common/src/sparse_matrix.rs:409-449):
+ * the entries (col, row, value) in (col, row) order; t_row_starts gets
+ * num_cols entries.  A counting sort by column over the rows in order gives
+ * that order (within a column the rows come out increasing).  Returns -1 if
+ * a column index is out of range. */
+int or_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
+                        const uint32_t* cols, const ofr* vals, uint32_t* t_row_starts,
+                        uint32_t* t_cols, ofr* t_vals) {
+  size_t* next = (size_t*)calloc(num_cols + 1, sizeof(size_t));
+  if (!next) return -2;
+  for (size_t i = 0; i < nnz; i++) {
+    if (cols[i] >= num_cols) {
+      free(next);
+      return -1;
+    }
+    next[cols[i] + 1]++;
+  }
+  for (size_t c = 0; c < num_cols; c++) next[c + 1] += next[c];
+  for (size_t c = 0; c < num_cols; c++) t_row_starts[c] = (uint32_t)next[c];
+  for (size_t r = 0; r < num_rows; r++) {
+    size_t b = row_starts[r], e = r + 1 < num_rows ? row_starts[r + 1] : nnz;
+    for (size_t i = b; i < e; i++) {
+      size_t k = next[cols[i]]++;
+      t_cols[k] = (uint32_t)r;
+      t_vals[k] = vals[i];
+    }
+  }
+  free(next);
+  return 0;
+}
+
 /* The R1CS map of run_zk_sumcheck_prover (whir_r1cs.rs:552-559) on one pair */
 static void r1cs_map(ofr a0, ofr a1, ofr b0, ofr b1, ofr c0, ofr c1, ofr e0, ofr e1, ofr o[3]) {
   ofr f0 = ofr_mul(e0, ofr_sub(ofr_mul(a0, b0), c0));

@@ -80,6 +80,10 @@ SIGNATURES = {
     "pkgpu_graph_destroy": [_vp],
     "pkgpu_sparse_mul": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp],
     "pkgpu_sparse_mul_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sparse_transpose": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sparse_transpose_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_multiply_transposed_by_eq_alpha_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp,
+                                                     _vp],
     "pkgpu_calculate_witness_bounds": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz,
                                        _vp, _vp, _vp, ctypes.POINTER(_sz)],
     "pkgpu_eq_evaluations": [_vp, _sz, _sz, _vp],
@@ -445,6 +449,48 @@ class SparseMatrix:
         _call("pkgpu_sparse_mul", self.num_rows, self.num_cols, rs, nnz, cols, vals,
               _ptr(zz) if zz.size else None, zz.shape[0], _ptr(out) if out.size else None)
         return out
+
+    def transpose(self) -> "SparseMatrix":
+        """SparseMatrix::transpose (sparse_matrix.rs:409-449): entries in
+        (column, row) order."""
+        _ensure_init()
+        trs = np.zeros(self.num_cols, np.uint32)
+        tcs = np.zeros(self.nnz, np.uint32)
+        tvs = np.zeros((self.nnz, 4), np.uint64)
+        rs, nnz, cols, vals = self._args()
+        _call("pkgpu_sparse_transpose", self.num_rows, self.num_cols, rs, nnz, cols, vals,
+              _ptr(trs) if trs.size else None, _ptr(tcs) if tcs.size else None,
+              _ptr(tvs) if tvs.size else None)
+        return SparseMatrix(self.num_cols, self.num_rows, trs, tcs, tvs)
+
+
+def transpose_r1cs_matrices(A: SparseMatrix, B: SparseMatrix, C: SparseMatrix):
+    """transpose_r1cs_matrices (provekit/common/src/utils/sumcheck.rs:194-201)."""
+    return A.transpose(), B.transpose(), C.transpose()
+
+
+def multiply_transposed_by_eq_alpha(at: SparseMatrix, bt: SparseMatrix, ct: SparseMatrix, alpha,
+                                    num_constraints: int) -> list:
+    """multiply_transposed_by_eq_alpha (sumcheck.rs:203-227): [At, Bt, Ct] times
+    eq(alpha, .) truncated to num_constraints entries (the transposed
+    matrices' column count)."""
+    _ensure_init()
+    al = _fr_array(alpha) if len(alpha) else np.zeros((0, 4), np.uint64)
+    dal = DeviceBuffer.from_array(al) if al.size else None
+    outs = []
+    for t in (at, bt, ct):
+        if t.num_cols != num_constraints:
+            raise ValueError("transposed matrix column count must equal the constraint count")
+        bufs = [DeviceBuffer.from_array(x) for x in (t.row_starts, t.cols, t.vals) if x.size]
+        it = iter(bufs)
+        rs = next(it).ptr if t.row_starts.size else None
+        cs = next(it).ptr if t.cols.size else None
+        vs = next(it).ptr if t.vals.size else None
+        out = DeviceBuffer(32 * max(t.num_rows, 1))
+        _call("pkgpu_multiply_transposed_by_eq_alpha_device", t.num_rows, t.num_cols, rs, t.nnz, cs, vs,
+              dal.ptr if dal is not None else None, al.shape[0], out.ptr, None)
+        outs.append(out.download(shape=(t.num_rows, 4)) if t.num_rows else np.zeros((0, 4), np.uint64))
+    return outs
 
 
 def calculate_witness_bounds(A: SparseMatrix, B: SparseMatrix, witness):

@@ -28,6 +28,7 @@ SOURCES = [
     "commit.hip",
     "poseidon2.hip",
     "sumcheck.hip",
+    "transpose.hip",
     "batch.hip",
     "serve.hip",
     "tune.cpp",

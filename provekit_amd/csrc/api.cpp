@@ -666,6 +666,56 @@ int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* ro
                  stream_or_thread(stream));
   });
 }
+int pkgpu_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                           size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                           uint32_t* t_row_starts, uint32_t* t_cols, uint64_t* t_vals) {
+  return guarded([&] {
+    check_csr(num_rows, row_starts, nnz, cols, vals);
+    REQ((num_cols == 0 || t_row_starts) && (nnz == 0 || (t_cols && t_vals)), pk::PK_ERR_ARG,
+        "transpose: null output");
+    validate_csr(num_rows, num_cols, row_starts, nnz, cols);
+    if (!num_cols) return;
+    hipStream_t s = pk::thread_stream();
+    std::vector<pk::Scratch> bufs;
+    pk::CsrDev A = upload_csr(num_rows, num_cols, row_starts, nnz, cols, vals, bufs, s);
+    Staged trs(sizeof(uint32_t) * num_cols, s), tcs(sizeof(uint32_t) * (nnz ? nnz : 1), s),
+        tvs(sizeof(fr_t) * (nnz ? nnz : 1), s);
+    pk::sparse_transpose_dev(A, trs.get<uint32_t>(), tcs.get<uint32_t>(), tvs.get<fr_t>(), s);
+    d2h(t_row_starts, trs.get<void>(), sizeof(uint32_t) * num_cols, s);
+    d2h(t_cols, tcs.get<void>(), sizeof(uint32_t) * nnz, s);
+    d2h(t_vals, tvs.get<void>(), sizeof(fr_t) * nnz, s);
+    sync(s);
+  });
+}
+int pkgpu_sparse_transpose_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                  size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                                  uint32_t* t_row_starts, uint32_t* t_cols, uint64_t* t_vals,
+                                  void* stream) {
+  return guarded([&] {
+    check_csr(num_rows, row_starts, nnz, cols, vals);
+    REQ((num_cols == 0 || t_row_starts) && (nnz == 0 || (t_cols && t_vals)), pk::PK_ERR_ARG,
+        "transpose: null output");
+    pk::sparse_transpose_dev(csr_view(num_rows, num_cols, row_starts, nnz, cols, vals),
+                             t_row_starts, t_cols, reinterpret_cast<fr_t*>(t_vals),
+                             stream_or_thread(stream));
+  });
+}
+int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num_cols,
+                                                 const uint32_t* t_row_starts, size_t nnz,
+                                                 const uint32_t* t_cols, const uint64_t* t_vals,
+                                                 const uint64_t* alpha, size_t m, uint64_t* out,
+                                                 void* stream) {
+  return guarded([&] {
+    check_csr(t_num_rows, t_row_starts, nnz, t_cols, t_vals);
+    REQ(m < 63 && t_num_cols <= (size_t(1) << m), pk::PK_ERR_SIZE,
+        "assertion failed: num_entries <= full_size");
+    REQ(t_num_rows == 0 || (out && (m == 0 || alpha)), pk::PK_ERR_ARG,
+        "transposed eq product: null pointer");
+    pk::transposed_eq_mul_dev(csr_view(t_num_rows, t_num_cols, t_row_starts, nnz, t_cols, t_vals),
+                              reinterpret_cast<const fr_t*>(alpha), (int)m,
+                              reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
 int pkgpu_calculate_witness_bounds(size_t num_rows, size_t num_cols, const uint32_t* a_rs,
                                    size_t a_nnz, const uint32_t* a_cols, const uint64_t* a_vals,
                                    const uint32_t* b_rs, size_t b_nnz, const uint32_t* b_cols,

@@ -25,6 +25,16 @@ Device& device_get(int id) {
   if (it != m.end()) return *it->second;
   auto d = std::make_unique<Device>();
   d->id = id;
+  // Scratch buffers come from the device's stream-ordered pool.  By default
+  // the pool hands freed memory back to the driver at every synchronisation,
+  // so the next call maps it again (a 128 MiB buffer costs about as much as
+  // the kernel that uses it); keep it cached instead.
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, id) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  (void)hipGetLastError();
   Device& ref = *d;
   m.emplace(id, std::move(d));
   return ref;

@@ -47,6 +47,14 @@ struct CsrDev {
 };
 
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s);
+// SparseMatrix::transpose: t_row_starts gets A.ncols entries, t_cols and
+// t_vals A.nnz (entries in (column, row) order).
+void sparse_transpose_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
+                          fr_t* t_vals, hipStream_t s);
+// out = At * eq(alpha, .) with the eq table truncated to At.ncols entries
+// (one matrix of multiply_transposed_by_eq_alpha); alpha: m device elements.
+void transposed_eq_mul_dev(const CsrDev& At, const fr_t* alpha, int m, fr_t* out,
+                           hipStream_t s);
 void hadamard_dev(const fr_t* a, const fr_t* b, fr_t* c, uint64_t n, uint64_t n_pad, hipStream_t s);
 void zero_dev(fr_t* p, uint64_t n, hipStream_t s);
 // eq(r, x) for x < num_entries (r: m device elements).

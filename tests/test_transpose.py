@@ -1,0 +1,194 @@
+"""CSR transpose (SparseMatrix::transpose, provekit/common/src/sparse_matrix.rs:409-449)
+and the product with eq(alpha, .) after the sumcheck (transpose_r1cs_matrices /
+multiply_transposed_by_eq_alpha, provekit/common/src/utils/sumcheck.rs:194-227).
+
+CPU: the oracle's transpose against a transpose written from the definition
+(dict of (col, row) -> value, sorted), the reference's own tests on its 3x4
+R1CS (sumcheck.rs:355-435: the rows of A recovered through At * e_i, and the
+expected At/Bt/Ct * eq(alpha) vectors), and the involution T(T(A)) = A.
+GPU: the C ABI bit-exact against the oracle on ragged shapes (empty rows and
+columns, one very long row, one very long column, no entries) and at 2^22
+rows with 1-4 entries per row."""
+import numpy as np
+import pytest
+
+from provekit_amd.inputs import random_fr
+from tests.util import P, mont, unmont
+
+
+def fe(v):
+    return mont(v % P)
+
+
+def frs(vals):
+    return np.array([fe(v) for v in vals], dtype=np.uint64).reshape(-1, 4)
+
+
+def ints(arr):
+    return [unmont(x) for x in np.asarray(arr).reshape(-1, 4)]
+
+
+def csr_from_rows(rows):
+    starts, cols, vals = [], [], []
+    for row in rows:
+        starts.append(len(cols))
+        for v, c in sorted(row, key=lambda t: t[1]):
+            cols.append(c)
+            vals.append(v)
+    return (np.array(starts, np.uint32), np.array(cols, np.uint32),
+            frs(vals) if vals else np.zeros((0, 4), np.uint64))
+
+
+def transpose_by_definition(num_rows, num_cols, rs, cs, vs):
+    """The transposed CSR from the entry set: row c lists (r, value) by r."""
+    ent = {}
+    nnz = len(cs)
+    for r in range(num_rows):
+        e = rs[r + 1] if r + 1 < num_rows else nnz
+        for i in range(rs[r], e):
+            ent[(int(cs[i]), r)] = vs[i]
+    trs, tcs, tvs = [], [], []
+    keys = sorted(ent)
+    k = 0
+    for c in range(num_cols):
+        trs.append(len(tcs))
+        while k < len(keys) and keys[k][0] == c:
+            tcs.append(keys[k][1])
+            tvs.append(ent[keys[k]])
+            k += 1
+    return (np.array(trs, np.uint32), np.array(tcs, np.uint32),
+            np.array(tvs, np.uint64).reshape(-1, 4))
+
+
+def random_csr(rng, rows, cols, per_row=(1, 5), seed=0):
+    k = rng.integers(per_row[0], per_row[1], size=rows)
+    w = per_row[1] - 1
+    cand = np.sort(rng.integers(0, cols, size=(rows, w)), axis=1)
+    keep = np.ones((rows, w), bool)
+    keep[:, 1:] = cand[:, 1:] != cand[:, :-1]
+    keep &= np.arange(w)[None, :] < k[:, None]
+    rs = np.zeros(rows, np.uint32)
+    if rows > 1:
+        rs[1:] = np.cumsum(keep.sum(axis=1))[:-1]
+    cs = cand[keep].astype(np.uint32)
+    pool = random_fr(min(max(cs.shape[0], 1), 1 << 16), seed=seed, stream="transpose")
+    vs = np.ascontiguousarray(np.resize(pool, (cs.shape[0], 4)))
+    return rs, cs, vs
+
+
+def ragged_cases():
+    rng = np.random.default_rng(11)
+    cases = []
+    # empty rows and columns, ncols not a power of two
+    rs, cs, vs = random_csr(rng, 300, 1000, (0, 4), seed=1)
+    cases.append(("empty-rows-cols", 300, 1000, rs, cs, vs))
+    # one very long row among short ones
+    rows = [[(1, c)] for c in range(5)] + [[(7 + c, c) for c in range(0, 3000, 2)]] + [[(2, 4)]] * 3
+    cases.append(("long-row", len(rows), 3000) + csr_from_rows(rows))
+    # one very long column (every row uses column 0, as a constant-one witness)
+    rows = [[(r + 1, 0), (r + 2, 1 + r % 50)] for r in range(5000)]
+    cases.append(("long-column", 5000, 51) + csr_from_rows(rows))
+    # no entries at all
+    cases.append(("no-entries", 4, 6, np.zeros(4, np.uint32), np.zeros(0, np.uint32),
+                  np.zeros((0, 4), np.uint64)))
+    # single entry in the last column
+    cases.append(("single", 1, 9) + csr_from_rows([[(5, 8)]]))
+    return cases
+
+
+# ---------------------------------------------------------------- CPU: oracle
+@pytest.mark.parametrize("case", ragged_cases(), ids=lambda c: c[0])
+def test_oracle_transpose_matches_definition(oracle, case):
+    _, rows, cols, rs, cs, vs = case
+    got = oracle.sparse_transpose(rows, cols, rs, cs, vs)
+    want = transpose_by_definition(rows, cols, rs, cs, vs)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w.reshape(g.shape))
+
+
+def test_oracle_transpose_involution(oracle):
+    rng = np.random.default_rng(3)
+    for rows, cols in [(1, 1), (17, 5), (1000, 1000), (4096, 300)]:
+        rs, cs, vs = random_csr(rng, rows, cols, seed=rows)
+        t = oracle.sparse_transpose(rows, cols, rs, cs, vs)
+        back = oracle.sparse_transpose(cols, rows, *t)
+        assert np.array_equal(back[0], rs) and np.array_equal(back[1], cs) and np.array_equal(back[2], vs)
+
+
+def test_oracle_transpose_rejects_bad_column(oracle):
+    with pytest.raises(ValueError):
+        oracle.sparse_transpose(1, 2, np.zeros(1, np.uint32), np.array([2], np.uint32), frs([1]))
+
+
+def test_oracle_r1cs_3x4_reference_tests(oracle, kats):
+    # sumcheck.rs:355-435 on make_test_r1cs
+    k = kats["r1cs_3x4"]
+    dense = {"A": [[1, 2, 0, 0], [0, 0, 3, 0], [1, 0, 0, 1]],
+             "B": [[0, 1, 0, 0], [2, 0, 0, 1], [0, 0, 4, 0]],
+             "C": [[0, 0, 1, 0], [0, 1, 0, 3], [2, 0, 0, 0]]}
+    eqa = oracle.eq_evals(frs(k["alpha"]), 3)
+    for name, key in (("A", "expected_a"), ("B", "expected_b"), ("C", "expected_c")):
+        rs, cs, vs = csr_from_rows(k[name])
+        trs, tcs, tvs = oracle.sparse_transpose(3, 4, rs, cs, vs)
+        for i in range(3):
+            e = frs([1 if j == i else 0 for j in range(3)])
+            assert ints(oracle.sparse_mul(4, 3, trs, tcs, tvs, e)) == dense[name][i], (name, i)
+        assert ints(oracle.sparse_mul(4, 3, trs, tcs, tvs, eqa)) == [v % P for v in k[key]], name
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ragged_cases(), ids=lambda c: c[0])
+def test_gpu_transpose_ragged(gpu, oracle, case):
+    name, rows, cols, rs, cs, vs = case
+    print(f"[transpose] {name} rows={rows} cols={cols} nnz={len(cs)}", flush=True)
+    t = gpu.SparseMatrix(rows, cols, rs, cs, vs).transpose()
+    want = oracle.sparse_transpose(rows, cols, rs, cs, vs)
+    assert (t.num_rows, t.num_cols) == (cols, rows)
+    assert np.array_equal(t.row_starts, want[0])
+    assert np.array_equal(t.cols, want[1])
+    assert np.array_equal(t.vals, want[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("log_rows", [10, 16, 22])
+def test_gpu_transpose_device_full_size(gpu, oracle, log_rows):
+    rng = np.random.default_rng(log_rows)
+    n = 1 << log_rows
+    rs, cs, vs = random_csr(rng, n, n, seed=log_rows)
+    nnz = cs.shape[0]
+    print(f"[transpose] device 2^{log_rows} nnz={nnz}", flush=True)
+    d = [gpu.DeviceBuffer.from_array(x) for x in (rs, cs, vs)]
+    trs, tcs, tvs = gpu.DeviceBuffer(4 * n), gpu.DeviceBuffer(4 * nnz), gpu.DeviceBuffer(32 * nnz)
+    gpu._call("pkgpu_sparse_transpose_device", n, n, d[0].ptr, nnz, d[1].ptr, d[2].ptr,
+              trs.ptr, tcs.ptr, tvs.ptr, None)
+    want = oracle.sparse_transpose(n, n, rs, cs, vs)
+    assert np.array_equal(trs.download(np.uint32), want[0])
+    assert np.array_equal(tcs.download(np.uint32), want[1])
+    assert np.array_equal(tvs.download(shape=(nnz, 4)), want[2])
+
+
+@pytest.mark.gpu
+def test_gpu_r1cs_3x4_multiply_transposed_by_eq_alpha(gpu, kats):
+    k = kats["r1cs_3x4"]
+    mats = [gpu.SparseMatrix(3, 4, *csr_from_rows(k[n])) for n in "ABC"]
+    at, bt, ct = gpu.transpose_r1cs_matrices(*mats)
+    assert (at.num_rows, at.num_cols) == (4, 3)
+    got = gpu.multiply_transposed_by_eq_alpha(at, bt, ct, frs(k["alpha"]), 3)
+    for g, key in zip(got, ("expected_a", "expected_b", "expected_c")):
+        assert ints(g) == [v % P for v in k[key]], key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,m", [(1000, 10), (1 << 16, 16), (50000, 16)])
+def test_gpu_multiply_transposed_by_eq_alpha_random(gpu, oracle, rows, m):
+    rng = np.random.default_rng(rows)
+    cols = 3000
+    mats = [gpu.SparseMatrix(rows, cols, *random_csr(rng, rows, cols, seed=i)) for i in range(3)]
+    alpha = random_fr(m, seed=rows, stream="alpha")
+    ts = gpu.transpose_r1cs_matrices(*mats)
+    got = gpu.multiply_transposed_by_eq_alpha(*ts, alpha, rows)
+    eqa = oracle.eq_evals(alpha, rows)
+    for g, A in zip(got, mats):
+        t = oracle.sparse_transpose(rows, cols, A.row_starts, A.cols, A.vals)
+        assert np.array_equal(g, oracle.sparse_mul(cols, rows, *t, eqa))
