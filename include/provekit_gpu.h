@@ -209,6 +209,39 @@ int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num
                                                  const uint32_t* t_cols, const uint64_t* t_vals,
                                                  const uint64_t* alpha, size_t m, uint64_t* out,
                                                  void* stream);
+/* ---------------------------------------------------- R1CS wire format */
+/* Device CSR filled by the decoders below (caller-allocated buffers). */
+typedef struct {
+  size_t num_rows, num_cols, nnz;
+  uint32_t* row_starts; /* device, num_rows entries (row r ends at r+1's start, the last at nnz) */
+  uint32_t* cols;       /* device, nnz entries */
+  uint64_t* vals;       /* device, nnz Fr (Montgomery form) */
+} pkgpu_csr;
+
+/* The postcard encodings (the `postcard` crate, Cargo.toml:124) of Interner
+ * (provekit/common/src/interner.rs:6-10 with utils/serde_ark.rs:11-30),
+ * SparseMatrix (provekit/common/src/sparse_matrix.rs:149-290: delta-coded
+ * columns) and R1CS (provekit/common/src/r1cs.rs:59-66), as
+ * postcard::to_allocvec writes them, decoded on the device.  Blobs are host
+ * memory.  Each call writes the sizes; it also fills the outputs when they are
+ * given (interner decode: out != NULL; matrix decode: any of m's pointers set;
+ * R1CS decode: interner != NULL, and then every mats[i] buffer of non-zero
+ * size), which must hold the sizes a first call reported.  Matrix values come
+ * out hydrated (InternedFieldElement -> Fr).  Malformed input
+ * (PKGPU_ERR_ARG): a bad or truncated varint, num_rows different from the
+ * row-start count, row starts not starting at 0, decreasing or past the value
+ * count, delta count != value count, a column >= num_cols, an id >= the
+ * interner size, an interner integer >= p (the reference's deserializer
+ * checks the first and last of these; the others would panic later or give an
+ * inconsistent matrix there).  Synchronises the stream. */
+int pkgpu_interner_decode_device(const uint8_t* blob, size_t len, uint64_t* out, size_t* count,
+                                 void* stream);
+int pkgpu_sparse_matrix_decode_device(const uint8_t* blob, size_t len, const uint64_t* interner,
+                                      size_t interner_len, pkgpu_csr* m, void* stream);
+int pkgpu_r1cs_decode_device(const uint8_t* blob, size_t len, size_t* num_public_inputs,
+                             uint64_t* interner, size_t* interner_len, pkgpu_csr mats[3],
+                             void* stream);
+
 /* calculate_witness_bounds (provekit/common/src/utils/sumcheck.rs:163-179):
  * a = A w, b = B w, c = a o b, each zero-padded to padded_len =
  * num_rows.next_power_of_two() (written to *padded_len; the out arrays must

@@ -69,6 +69,16 @@ void or_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
 int or_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts, size_t nnz,
                         const uint32_t* cols, const ofr* vals, uint32_t* t_row_starts,
                         uint32_t* t_cols, ofr* t_vals);
+/* ---- wire format (postcard R1CS / Interner / SparseMatrix) ---- */
+int or_varint_decode(const uint8_t* b, size_t len, int max_bytes, int bits, uint64_t* out,
+                     size_t* used);
+int or_interner_decode(const uint8_t* blob, size_t len, ofr* out, size_t cap, size_t* count);
+int or_sparse_matrix_decode(const uint8_t* blob, size_t len, const ofr* interner,
+                            size_t interner_len, size_t cap_rows, size_t cap_nnz, uint32_t* rs,
+                            uint32_t* cols, ofr* vals, size_t* num_rows, size_t* num_cols,
+                            size_t* nnz);
+int or_r1cs_decode(const uint8_t* blob, size_t len, size_t sizes[11], uint32_t* const rs[3],
+                   uint32_t* const cols[3], ofr* const vals[3], const size_t caps[6]);
 void or_sumcheck_r1cs_round(ofr* a, ofr* b, ofr* c, ofr* eq, size_t size, const ofr* fold,
                             ofr out[3]);
 ofr or_eval_cubic(const ofr poly[4], ofr x);

@@ -66,6 +66,11 @@ def lib():
             "or_calculate_eq": ([_vp, _vp, _sz], OFr),
             "or_sparse_mul": ([_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp], None),
             "or_sparse_transpose": ([_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp], _i),
+            "or_varint_decode": ([_vp, _sz, _i, _i, ctypes.POINTER(_u64), ctypes.POINTER(_sz)], _i),
+            "or_interner_decode": ([_vp, _sz, _vp, _sz, ctypes.POINTER(_sz)], _i),
+            "or_sparse_matrix_decode": ([_vp, _sz, _vp, _sz, _sz, _sz, _vp, _vp, _vp,
+                                         ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)], _i),
+            "or_r1cs_decode": ([_vp, _sz, _vp, _vp, _vp, _vp, _vp], _i),
             "or_sumcheck_r1cs_round": ([_vp, _vp, _vp, _vp, _sz, _vp, _vp], None),
             "or_eval_cubic": ([_vp, OFr], OFr),
             "or_sumcheck_prove": ([_vp, _vp, _vp, _sz, _vp, _u64, _vp, _vp, _vp, _vp], _i),
@@ -342,6 +347,73 @@ def sparse_transpose(num_rows: int, num_cols: int, row_starts, cols, vals):
     if lib().or_sparse_transpose(num_rows, num_cols, _p(rs), nnz, _p(cs), _p(vs), _p(trs), _p(tcs), _p(tvs)):
         raise ValueError("column index out of range")
     return trs, tcs, tvs
+
+
+class WireError(ValueError):
+    pass
+
+
+def varint_decode(data: bytes, max_bytes: int, bits: int):
+    """(value, bytes used) of the LEB128 varint at the start of data."""
+    b = np.frombuffer(data, np.uint8)
+    v, used = _u64(0), _sz(0)
+    if lib().or_varint_decode(_p(b), b.size, max_bytes, bits, ctypes.byref(v), ctypes.byref(used)):
+        raise WireError("bad varint")
+    return int(v.value), int(used.value)
+
+
+def interner_decode(blob: bytes) -> np.ndarray:
+    b = np.frombuffer(blob, np.uint8)
+    n = _sz(0)
+    if lib().or_interner_decode(_p(b), b.size, None, 0, ctypes.byref(n)):
+        raise WireError("malformed interner")
+    out = np.zeros((n.value, 4), np.uint64)
+    if lib().or_interner_decode(_p(b), b.size, _p(out), n.value, ctypes.byref(n)):
+        raise WireError("malformed interner")
+    return out
+
+
+def sparse_matrix_decode(blob: bytes, interner):
+    """(num_rows, num_cols, row_starts, cols, vals) of a postcard SparseMatrix."""
+    b = np.frombuffer(blob, np.uint8)
+    it = _a(interner).reshape(-1, 4)
+    nr, nc, nz = _sz(0), _sz(0), _sz(0)
+    sz = (ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nz))
+    if lib().or_sparse_matrix_decode(_p(b), b.size, _p(it), it.shape[0], 0, 0, None, None, None, *sz):
+        raise WireError("malformed sparse matrix")
+    rs = np.zeros(max(nr.value, 1), np.uint32)
+    cs = np.zeros(max(nz.value, 1), np.uint32)
+    vs = np.zeros((max(nz.value, 1), 4), np.uint64)
+    if lib().or_sparse_matrix_decode(_p(b), b.size, _p(it), it.shape[0], nr.value, nz.value,
+                                     _p(rs), _p(cs), _p(vs), *sz):
+        raise WireError("malformed sparse matrix")
+    return nr.value, nc.value, rs[: nr.value], cs[: nz.value], vs[: nz.value]
+
+
+def r1cs_decode(blob: bytes) -> dict:
+    b = np.frombuffer(blob, np.uint8)
+    sizes = np.zeros(11, np.uint64)
+    nulls = (ctypes.c_void_p * 3)()
+    if lib().or_r1cs_decode(_p(b), b.size, _p(sizes), nulls, nulls, nulls, None):
+        raise WireError("malformed r1cs")
+    mats = []
+    rs_p, cs_p, vs_p = (ctypes.c_void_p * 3)(), (ctypes.c_void_p * 3)(), (ctypes.c_void_p * 3)()
+    caps = np.zeros(6, np.uint64)
+    keep = []
+    for m in range(3):
+        nr, nz = int(sizes[2 + 3 * m]), int(sizes[4 + 3 * m])
+        arrs = (np.zeros(max(nr, 1), np.uint32), np.zeros(max(nz, 1), np.uint32),
+                np.zeros((max(nz, 1), 4), np.uint64))
+        keep.append(arrs)
+        rs_p[m], cs_p[m], vs_p[m] = _p(arrs[0]), _p(arrs[1]), _p(arrs[2])
+        caps[2 * m], caps[2 * m + 1] = nr, nz
+    if lib().or_r1cs_decode(_p(b), b.size, _p(sizes), rs_p, cs_p, vs_p, _p(caps)):
+        raise WireError("malformed r1cs")
+    for m in range(3):
+        nr, nc, nz = (int(x) for x in sizes[2 + 3 * m: 5 + 3 * m])
+        mats.append((nr, nc, keep[m][0][:nr], keep[m][1][:nz], keep[m][2][:nz]))
+    return {"num_public_inputs": int(sizes[0]), "interner_len": int(sizes[1]),
+            "a": mats[0], "b": mats[1], "c": mats[2]}
 
 
 def sumcheck_round(a, b, c, eq, fold=None) -> np.ndarray:

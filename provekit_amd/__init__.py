@@ -80,6 +80,9 @@ SIGNATURES = {
     "pkgpu_graph_destroy": [_vp],
     "pkgpu_sparse_mul": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp],
     "pkgpu_sparse_mul_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_interner_decode_device": [_vp, _sz, _vp, ctypes.POINTER(_sz), _vp],
+    "pkgpu_sparse_matrix_decode_device": [_vp, _sz, _vp, _sz, _vp, _vp],
+    "pkgpu_r1cs_decode_device": [_vp, _sz, ctypes.POINTER(_sz), _vp, ctypes.POINTER(_sz), _vp, _vp],
     "pkgpu_sparse_transpose": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_sparse_transpose_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_multiply_transposed_by_eq_alpha_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp,
@@ -491,6 +494,73 @@ def multiply_transposed_by_eq_alpha(at: SparseMatrix, bt: SparseMatrix, ct: Spar
               dal.ptr if dal is not None else None, al.shape[0], out.ptr, None)
         outs.append(out.download(shape=(t.num_rows, 4)) if t.num_rows else np.zeros((0, 4), np.uint64))
     return outs
+
+
+class PkCsr(ctypes.Structure):
+    """pkgpu_csr (include/provekit_gpu.h)."""
+    _fields_ = [("num_rows", _sz), ("num_cols", _sz), ("nnz", _sz), ("row_starts", _vp),
+                ("cols", _vp), ("vals", _vp)]
+
+
+def _csr_download(c: "PkCsr", bufs) -> SparseMatrix:
+    rs = bufs[0].download(np.uint32)[: c.num_rows] if c.num_rows else np.zeros(0, np.uint32)
+    cs = bufs[1].download(np.uint32)[: c.nnz] if c.nnz else np.zeros(0, np.uint32)
+    vs = bufs[2].download(shape=(-1, 4))[: c.nnz] if c.nnz else np.zeros((0, 4), np.uint64)
+    return SparseMatrix(c.num_rows, c.num_cols, rs, cs, vs)
+
+
+def _csr_alloc(c: "PkCsr"):
+    bufs = [DeviceBuffer(4 * max(c.num_rows, 1)), DeviceBuffer(4 * max(c.nnz, 1)),
+            DeviceBuffer(32 * max(c.nnz, 1))]
+    c.row_starts, c.cols, c.vals = bufs[0].ptr, bufs[1].ptr, bufs[2].ptr
+    return bufs
+
+
+def decode_interner(blob: bytes) -> np.ndarray:
+    """Interner (interner.rs:6-10, postcard + serde_ark) -> (n, 4) Montgomery Fr."""
+    _ensure_init()
+    b = np.frombuffer(blob, np.uint8)
+    n = _sz()
+    _call("pkgpu_interner_decode_device", _ptr(b) if b.size else None, b.size, None, ctypes.byref(n), None)
+    if not n.value:
+        return np.zeros((0, 4), np.uint64)
+    out = DeviceBuffer(32 * n.value)
+    _call("pkgpu_interner_decode_device", _ptr(b), b.size, out.ptr, ctypes.byref(n), None)
+    return out.download(shape=(n.value, 4))
+
+
+def decode_sparse_matrix(blob: bytes, interner) -> SparseMatrix:
+    """SparseMatrix (sparse_matrix.rs:149-290, postcard) decoded on the device,
+    values hydrated through `interner` ((n, 4) Montgomery Fr)."""
+    _ensure_init()
+    b = np.frombuffer(blob, np.uint8)
+    it = np.ascontiguousarray(interner, dtype=np.uint64).reshape(-1, 4)
+    dit = DeviceBuffer.from_array(it) if it.size else None
+    c = PkCsr()
+    _call("pkgpu_sparse_matrix_decode_device", _ptr(b) if b.size else None, b.size,
+          dit.ptr if dit else None, it.shape[0], ctypes.byref(c), None)
+    bufs = _csr_alloc(c)
+    _call("pkgpu_sparse_matrix_decode_device", _ptr(b) if b.size else None, b.size,
+          dit.ptr if dit else None, it.shape[0], ctypes.byref(c), None)
+    return _csr_download(c, bufs)
+
+
+def decode_r1cs(blob: bytes) -> dict:
+    """R1CS (r1cs.rs:59-66, postcard) decoded on the device: num_public_inputs,
+    the interner ((n, 4) Montgomery Fr) and the hydrated a, b, c."""
+    _ensure_init()
+    b = np.frombuffer(blob, np.uint8)
+    npub, ni = _sz(), _sz()
+    mats = (PkCsr * 3)()
+    args = (_ptr(b) if b.size else None, b.size, ctypes.byref(npub))
+    _call("pkgpu_r1cs_decode_device", *args, None, ctypes.byref(ni), mats, None)
+    inter = DeviceBuffer(32 * max(ni.value, 1))
+    bufs = [_csr_alloc(mats[i]) for i in range(3)]
+    _call("pkgpu_r1cs_decode_device", *args, inter.ptr, ctypes.byref(ni), mats, None)
+    return {"num_public_inputs": npub.value,
+            "interner": inter.download(shape=(-1, 4))[: ni.value],
+            "a": _csr_download(mats[0], bufs[0]), "b": _csr_download(mats[1], bufs[1]),
+            "c": _csr_download(mats[2], bufs[2])}
 
 
 def calculate_witness_bounds(A: SparseMatrix, B: SparseMatrix, witness):

@@ -29,6 +29,7 @@ SOURCES = [
     "poseidon2.hip",
     "sumcheck.hip",
     "transpose.hip",
+    "wire.hip",
     "batch.hip",
     "serve.hip",
     "tune.cpp",

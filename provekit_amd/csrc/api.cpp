@@ -10,6 +10,7 @@
 #include "batch.hpp"
 #include "serve.hpp"
 #include "sumcheck.hpp"
+#include "wire.hpp"
 
 using pk::Error;
 using pk::fr_t;
@@ -714,6 +715,52 @@ int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num
     pk::transposed_eq_mul_dev(csr_view(t_num_rows, t_num_cols, t_row_starts, nnz, t_cols, t_vals),
                               reinterpret_cast<const fr_t*>(alpha), (int)m,
                               reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
+int pkgpu_interner_decode_device(const uint8_t* blob, size_t len, uint64_t* out, size_t* count,
+                                 void* stream) {
+  return guarded([&] {
+    REQ(count && (blob || !len), pk::PK_ERR_ARG, "interner decode: null pointer");
+    *count = pk::interner_decode_dev(blob, len, reinterpret_cast<fr_t*>(out),
+                                     stream_or_thread(stream));
+  });
+}
+int pkgpu_sparse_matrix_decode_device(const uint8_t* blob, size_t len, const uint64_t* interner,
+                                      size_t interner_len, pkgpu_csr* m, void* stream) {
+  return guarded([&] {
+    REQ(m && (blob || !len), pk::PK_ERR_ARG, "sparse matrix decode: null pointer");
+    const bool fill = m->row_starts || m->cols || m->vals;
+    REQ(!fill || interner_len == 0 || interner, pk::PK_ERR_ARG,
+        "sparse matrix decode: null interner");
+    pk::WireCsrOut o{m->row_starts, m->cols, reinterpret_cast<fr_t*>(m->vals)};
+    const pk::WireCsrSizes z = pk::sparse_matrix_decode_dev(
+        blob, len, reinterpret_cast<const fr_t*>(interner), interner_len, fill ? &o : nullptr,
+        stream_or_thread(stream));
+    m->num_rows = z.num_rows;
+    m->num_cols = z.num_cols;
+    m->nnz = z.nnz;
+  });
+}
+int pkgpu_r1cs_decode_device(const uint8_t* blob, size_t len, size_t* num_public_inputs,
+                             uint64_t* interner, size_t* interner_len, pkgpu_csr mats[3],
+                             void* stream) {
+  return guarded([&] {
+    REQ(num_public_inputs && interner_len && mats && (blob || !len), pk::PK_ERR_ARG,
+        "r1cs decode: null pointer");
+    pk::WireCsrOut outs[3];
+    for (int i = 0; i < 3; i++)
+      outs[i] = pk::WireCsrOut{mats[i].row_starts, mats[i].cols,
+                               reinterpret_cast<fr_t*>(mats[i].vals)};
+    const pk::R1csSizes z = pk::r1cs_decode_dev(blob, len, reinterpret_cast<fr_t*>(interner),
+                                                interner ? outs : nullptr,
+                                                stream_or_thread(stream));
+    *num_public_inputs = z.num_public_inputs;
+    *interner_len = z.interner_len;
+    for (int i = 0; i < 3; i++) {
+      mats[i].num_rows = z.mats[i].num_rows;
+      mats[i].num_cols = z.mats[i].num_cols;
+      mats[i].nnz = z.mats[i].nnz;
+    }
   });
 }
 int pkgpu_calculate_witness_bounds(size_t num_rows, size_t num_cols, const uint32_t* a_rs,
