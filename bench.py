@@ -451,6 +451,93 @@ def workload_pow(pk, hip, s, args):
                                        "sample": f"one solve at {cpu_bits:g} bits: nonce {cpu_n} in {cpu_s:.2f} s"}})
 
 
+def _leb128(a):
+    """Postcard varints of the u64 array a, concatenated (vectorised)."""
+    a = np.asarray(a, dtype=np.uint64)
+    n = np.ones(a.shape, np.int64)
+    for k in range(1, 10):
+        n += (a >= (np.uint64(1) << np.uint64(7 * k))).astype(np.int64)
+    out = np.zeros(int(n.sum()), np.uint8)
+    pos = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    for k in range(10):
+        m = n > k
+        byte = ((a[m] >> np.uint64(7 * k)) & np.uint64(0x7F)).astype(np.uint8)
+        byte |= np.where(n[m] > k + 1, 0x80, 0).astype(np.uint8)
+        out[pos[m] + k] = byte
+    return out.tobytes()
+
+
+def workload_wire(pk, hip, s, args):
+    """R1CS wire format: the postcard bytes of an R1CS with 2^m constraints
+    (1-4 entries per row at uniform columns, 16 interned values) decoded into
+    hydrated device CSR, host bytes in, device matrices out."""
+    import ctypes
+    from oracle import pyoracle
+
+    m = args.log_m
+    n = 1 << m
+    rng = np.random.default_rng(args.seed)
+    values = [int(x) for x in rng.integers(0, 2**62, size=16)]
+    v1 = lambda x: _leb128([x])  # noqa: E731
+    payload = (16).to_bytes(8, "little") + b"".join(v.to_bytes(32, "little") for v in values)
+    parts = [v1(3), v1(len(payload)), payload]
+    nnz = []
+    for _ in range(3):
+        k = rng.integers(1, 5, size=n)
+        cand = np.sort(rng.integers(0, n, size=(n, 4)), axis=1)
+        keep = np.ones((n, 4), bool)
+        keep[:, 1:] = cand[:, 1:] != cand[:, :-1]
+        keep &= np.arange(4)[None, :] < k[:, None]
+        rs = np.zeros(n, np.int64)
+        rs[1:] = np.cumsum(keep.sum(axis=1))[:-1]
+        cs = cand[keep].astype(np.int64)
+        d = cs.copy()
+        d[1:] -= cs[:-1]
+        d[rs] = cs[rs]  # every row is non-empty here
+        ids = rng.integers(0, 16, size=cs.shape[0])
+        parts += [v1(n), v1(n), v1(n), _leb128(rs), v1(cs.shape[0]), _leb128(d), v1(cs.shape[0]), _leb128(ids)]
+        nnz.append(int(cs.shape[0]))
+    blob = b"".join(parts)
+    b = np.frombuffer(blob, np.uint8)
+    npub, ni = ctypes.c_size_t(), ctypes.c_size_t()
+    mats = (pk.PkCsr * 3)()
+    pk._call("pkgpu_r1cs_decode_device", pk._ptr(b), b.size, ctypes.byref(npub), None, ctypes.byref(ni), mats, s)
+    inter = pk.DeviceBuffer(32 * ni.value)
+    bufs = [pk._csr_alloc(mats[i]) for i in range(3)]
+
+    def decode():
+        pk._call("pkgpu_r1cs_decode_device", pk._ptr(b), b.size, ctypes.byref(npub), inter.ptr,
+                 ctypes.byref(ni), mats, s)
+
+    for _ in range(max(1, args.warmup)):
+        decode()
+    steps = max(1, min(args.steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        decode()
+    hip.sync(s)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    got = pk._csr_download(mats[2], bufs[2])
+    want = pyoracle.r1cs_decode(blob)
+    cpu_s = want["fill_seconds"]
+    ok = bool(np.array_equal(got.cols, want["c"][3]) and np.array_equal(got.vals, want["c"][4]) and
+              np.array_equal(got.row_starts, want["c"][2]))
+    total = sum(nnz)
+    out_bytes = sum(4 * n + 36 * k for k in nnz)
+    return base_line(f"R1CS entries/s decoded (postcard wire format, 2^{m} constraints)", total / (ms / 1e3),
+                     "entries/s", steps, args.warmup, ms, True, "u8",
+                     {"workload": f"postcard R1CS, 2^{m} constraints x 3 matrices, 1-4 entries per row, "
+                                  f"16 interned values ({len(blob) / 2**20:.1f} MiB of bytes, {total} entries); "
+                                  "host bytes to hydrated device CSR"},
+                     {"blob_bytes": len(blob), "blob_GB_per_s": len(blob) / (ms / 1e3) / 1e9,
+                      "device_bytes_written": out_bytes, "matches_oracle": ok,
+                      "note": "the time includes the host-to-device copy of the bytes (pageable host memory) "
+                              "and one synchronisation per header read",
+                      "cpu_baseline": {"value": total / cpu_s, "unit": "entries/s", "cores": 1, "kind": "port",
+                                       "sample": f"the same bytes decoded once by the sequential C restatement "
+                                                 f"in {cpu_s:.2f} s (its filling pass, after a sizes pass)"}})
+
+
 def workload_sumcheck(pk, hip, s, args):
     """R1CS of 2^m constraints (1..4 nonzeros per row at uniform columns,
     uniform Fr values, z[0] = 1): a, b, c = A z, B z, C z, then the whole
@@ -585,10 +672,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy kernel (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--workload", default="commit",
-                    choices=["commit", "ntt", "p2merkle", "serving", "sumcheck", "pow"],
+                    choices=["commit", "ntt", "p2merkle", "serving", "sumcheck", "pow", "wire"],
                     help="commit (default: 2^20-coefficient Skyscraper commit) | ntt (forward NTT 2^16) | "
                          "p2merkle (Poseidon2 tree over 2^24 leaves) | serving (Poisson R1CS jobs) | "
-                         "sumcheck (2^m R1CS SpMV + sumcheck) | pow (Skyscraper proof-of-work grinding)")
+                         "sumcheck (2^m R1CS SpMV + sumcheck) | pow (Skyscraper proof-of-work grinding) | "
+                         "wire (postcard R1CS of 2^m constraints decoded to device CSR)")
     ap.add_argument("--pow-bits", type=float, default=30.0, help="pow workload: difficulty in bits")
     ap.add_argument("--log-m", type=int, default=22, help="sumcheck workload: 2^m constraints")
     ap.add_argument("--jobs", type=int, default=64)
@@ -610,7 +698,7 @@ def main():
     s = hip.stream()
     if args.workload != "commit":
         fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving,
-              "sumcheck": workload_sumcheck, "pow": workload_pow}[args.workload]
+              "sumcheck": workload_sumcheck, "pow": workload_pow, "wire": workload_wire}[args.workload]
         print(json.dumps(fn(pk, hip, s, args)), flush=True)
         return
 

@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
+import time
 from typing import Optional
 
 import numpy as np
@@ -407,13 +408,15 @@ def r1cs_decode(blob: bytes) -> dict:
         keep.append(arrs)
         rs_p[m], cs_p[m], vs_p[m] = _p(arrs[0]), _p(arrs[1]), _p(arrs[2])
         caps[2 * m], caps[2 * m + 1] = nr, nz
+    t0 = time.perf_counter()
     if lib().or_r1cs_decode(_p(b), b.size, _p(sizes), rs_p, cs_p, vs_p, _p(caps)):
         raise WireError("malformed r1cs")
+    fill_s = time.perf_counter() - t0
     for m in range(3):
         nr, nc, nz = (int(x) for x in sizes[2 + 3 * m: 5 + 3 * m])
         mats.append((nr, nc, keep[m][0][:nr], keep[m][1][:nz], keep[m][2][:nz]))
     return {"num_public_inputs": int(sizes[0]), "interner_len": int(sizes[1]),
-            "a": mats[0], "b": mats[1], "c": mats[2]}
+            "a": mats[0], "b": mats[1], "c": mats[2], "fill_seconds": fill_s}
 
 
 def sumcheck_round(a, b, c, eq, fold=None) -> np.ndarray:
