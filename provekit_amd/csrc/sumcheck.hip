@@ -119,40 +119,53 @@ __global__ __launch_bounds__(256) void k_eq_combine(const fr_t* hi, const fr_t* 
 constexpr int kRoundThreads = 256;
 constexpr unsigned kMaxRoundBlocks = 1024;
 
-// Workgroup sum of three values per lane; lane 0 writes them to dst.
-__device__ __forceinline__ void block_sum3(fr_t v0, fr_t v1, fr_t v2, fr_t* dst) {
-  __shared__ fr_t sh[3][kRoundThreads];
+// Workgroup sum of NV values per lane; lane 0 writes them (canonical) to dst.
+template <int NV>
+__device__ __forceinline__ void block_sum(const fr_t (&v)[NV], fr_t* dst) {
+  __shared__ fr_t sh[NV][kRoundThreads];
   const int t = threadIdx.x;
-  sh[0][t] = v0;
-  sh[1][t] = v1;
-  sh[2][t] = v2;
+#pragma unroll
+  for (int q = 0; q < NV; q++) sh[q][t] = v[q];
   __syncthreads();
   for (int w = kRoundThreads / 2; w > 0; w >>= 1) {
     if (t < w) {
 #pragma unroll
-      for (int q = 0; q < 3; q++) sh[q][t] = fr_add(sh[q][t], sh[q][t + w]);
+      for (int q = 0; q < NV; q++) sh[q][t] = fr_add(sh[q][t], sh[q][t + w]);
     }
     __syncthreads();
   }
   if (t == 0) {
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-      fr_t s = sh[q][0];
-      fr_canon(s);
-      fr_store(dst + q, s);
+    for (int q = 0; q < NV; q++) {
+      fr_t x = sh[q][0];
+      fr_canon(x);
+      fr_store(dst + q, x);
     }
   }
+}
+__device__ __forceinline__ void block_sum3(fr_t v0, fr_t v1, fr_t v2, fr_t* dst) {
+  const fr_t v[3] = {v0, v1, v2};
+  block_sum<3>(v, dst);
 }
 
 // One sumcheck_fold_map_reduce call.  fold == nullptr: map over the halves of
 // size; else fold the quarters in place with *fold first, map over (p0, p1).
+// With WITH_CLAIM (round 0 of the plain mode, no fold) each workgroup also
+// writes sum eq1 (a1 b1 - c1) over its high halves as a fourth partial: the
+// claim is h(0) + h(1) = sum_x eq (a b - c), so no separate pass over the
+// four MLEs is needed for it.
+__device__ __forceinline__ fr_t r1cs_term(const fr_t& e, const fr_t& a, const fr_t& b, const fr_t& c) {
+  return fr_mul(e, fr_sub(fr_mul(a, b), c));
+}
+
+template <bool WITH_CLAIM>
 __global__ __launch_bounds__(kRoundThreads) void k_sc_round(fr_t* a, fr_t* b, fr_t* c, fr_t* eq,
                                                             uint64_t size, const fr_t* fold,
                                                             fr_t* partials) {
   const uint64_t h = fold ? size / 4 : size / 2;
   fr_t f;
   if (fold) f = fr_load(fold);
-  fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero();
+  fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero(), s3 = fr_zero();
   fr_t* mle[4] = {a, b, c, eq};
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < h;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -175,7 +188,7 @@ __global__ __launch_bounds__(kRoundThreads) void k_sc_round(fr_t* a, fr_t* b, fr
     }
     // [eq0 (a0 b0 - c0), (2eq0 - eq1)((2a0 - a1)(2b0 - b1) - (2c0 - c1)),
     //  (eq1 - eq0)(a1 - a0)(b1 - b0)]
-    const fr_t f0 = fr_mul(lo[3], fr_sub(fr_mul(lo[0], lo[1]), lo[2]));
+    const fr_t f0 = r1cs_term(lo[3], lo[0], lo[1], lo[2]);
     const fr_t ea = fr_sub(fr_add(lo[3], lo[3]), hi[3]);
     const fr_t aa = fr_sub(fr_add(lo[0], lo[0]), hi[0]);
     const fr_t bb = fr_sub(fr_add(lo[1], lo[1]), hi[1]);
@@ -185,8 +198,14 @@ __global__ __launch_bounds__(kRoundThreads) void k_sc_round(fr_t* a, fr_t* b, fr
     s0 = fr_add(s0, f0);
     s1 = fr_add(s1, fm1);
     s2 = fr_add(s2, finf);
+    if (WITH_CLAIM) s3 = fr_add(s3, r1cs_term(hi[3], hi[0], hi[1], hi[2]));
   }
-  block_sum3(s0, s1, s2, partials + 3 * blockIdx.x);
+  if (WITH_CLAIM) {
+    const fr_t v[4] = {s0, s1, s2, s3};
+    block_sum<4>(v, partials + 4 * blockIdx.x);
+  } else {
+    block_sum3(s0, s1, s2, partials + 3 * blockIdx.x);
+  }
 }
 
 // sum_x eq(x) (a(x) b(x) - c(x)) partials: the claimed sum of the plain mode.
@@ -316,39 +335,32 @@ __global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_
   }
 }
 
-// Round idx: h = (h(0), h(-1), h(inf)) = the sum of the np workgroup
-// partials of k_sc_round; form the cubic, check c0 + c0 + c1 + c2 + c3 = S,
-// the transcript round step (the new state is alpha_idx), S = cubic(alpha).
-__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* partials, unsigned np,
-                                                    int idx) {
-  __shared__ fr_t red[3][64];
+// The end of round idx, run by every thread of the calling workgroup (it
+// holds barriers): h0, hm1, hinf are the round's sums (read on thread 0).
+// With `claim_hi` (round 0 of the plain mode) thread 0 first takes
+// S = claim = h(0) + h(1) and absorbs it (the order of k_sc_claim_finalize).
+// Then: form the cubic, check c0 + c0 + c1 + c2 + c3 = S, the transcript
+// round step (the new state is alpha_idx), S = cubic(alpha).  Returns
+// alpha_idx on thread 0.
+__device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t& hm1,
+                               const fr_t& hinf, const fr_t* claim_hi) {
   __shared__ fr_t co[4];
   __shared__ fr_t hh[2];
+  __shared__ fr_t state;
   const int t = threadIdx.x;
-  {
-    fr_t s0 = fr_zero(), s1 = fr_zero(), s2 = fr_zero();
-    for (unsigned i = t; i < np; i += 64) {
-      s0 = fr_add(s0, fr_load(partials + 3 * i));
-      s1 = fr_add(s1, fr_load(partials + 3 * i + 1));
-      s2 = fr_add(s2, fr_load(partials + 3 * i + 2));
-    }
-    red[0][t] = s0;
-    red[1][t] = s1;
-    red[2][t] = s2;
-    __syncthreads();
-    for (int w = 32; w > 0; w >>= 1) {
-      if (t < w) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) red[q][t] = fr_add(red[q][t], red[q][t + w]);
-      }
-      __syncthreads();
-    }
-  }
   if (t == 0) {
+    fr_t stt = st->state;
+    if (claim_hi) {
+      fr_t cl = fr_add(h0, *claim_hi);
+      fr_canon(cl);
+      st->S = cl;
+      st->rho = fr_zero();
+      stt = sky_compress(stt, cl);
+    }
+    state = stt;
     fr_t g[4] = {fr_zero(), fr_zero(), fr_zero(), fr_zero()};
     if (st->mode == 1) blinding_coeffs(st, idx, g);
     const fr_t S = st->S, rho = st->rho;
-    const fr_t h0 = red[0][0], hm1 = red[1][0], hinf = red[2][0];
     fr_t c0 = fr_add(h0, fr_mul(rho, g[0]));
     const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
     const fr_t at_m1 = fr_add(hm1, fr_mul(rho, gm1));
@@ -378,21 +390,137 @@ __global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* par
     st->coeffs[idx][3] = c3;
   }
   __syncthreads();
-  // round step: lane 0 hashes (state, c0), lane 1 (c2, c3), then one more.
-  // The operands are picked per lane before one call, so both lanes run the
-  // same instructions at once (two calls on divergent branches would run one
-  // after the other).
+  // round step: thread 0 hashes (state, c0), thread 1 (c2, c3), then one
+  // more.  The operands are picked per lane before one call, so both lanes
+  // run the same instructions at once (two calls on divergent branches would
+  // run one after the other).
   if (t < 2) {
-    const fr_t x = t == 0 ? st->state : co[2];
+    const fr_t x = t == 0 ? state : co[2];
     const fr_t y = t == 0 ? co[0] : co[3];
     hh[t] = sky_compress(x, y);
   }
   __syncthreads();
+  fr_t alpha = fr_zero();
   if (t == 0) {
-    const fr_t s1 = sky_compress(hh[0], hh[1]);
-    st->alpha[idx] = s1;
-    st->state = s1;
-    st->S = cubic(st->coeffs[idx], s1);
+    alpha = sky_compress(hh[0], hh[1]);
+    st->alpha[idx] = alpha;
+    st->state = alpha;
+    const fr_t cf[4] = {co[0], co[1], co[2], co[3]};
+    st->S = cubic(cf, alpha);
+  }
+  return alpha;
+}
+
+// Sum of np partials of NV values (stride NV) over the 64 lanes of one wave;
+// the result is read on lane 0.
+template <int NV>
+__device__ __forceinline__ void wave_sum_partials(const fr_t* partials, unsigned np,
+                                                  fr_t (&out)[NV]) {
+  __shared__ fr_t red[NV][64];
+  const int t = threadIdx.x;
+  fr_t acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) acc[q] = fr_zero();
+  for (unsigned i = t; i < np; i += 64) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] = fr_add(acc[q], fr_load(partials + NV * i + q));
+  }
+#pragma unroll
+  for (int q = 0; q < NV; q++) red[q][t] = acc[q];
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < NV; q++) red[q][t] = fr_add(red[q][t], red[q][t + w]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NV; q++) out[q] = red[q][0];
+}
+
+// Round idx after k_sc_round: sum the np workgroup partials, then
+// finalize_round.  NV = 4: round 0 of the plain mode, whose partials also
+// carry the high-half sums of the claim.
+template <int NV>
+__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* partials, unsigned np,
+                                                    int idx) {
+  fr_t sums[NV];
+  wave_sum_partials<NV>(partials, np, sums);
+  const fr_t hi = sums[NV - 1];
+  finalize_round(st, idx, sums[0], sums[1], sums[2], NV == 4 ? &hi : nullptr);
+}
+
+// The last rounds, from an input of cur <= kTailElems elements down to 2, in
+// one workgroup: fold + map (at most one pair per thread), workgroup sums,
+// finalize_round, then the next round, with no launch in between.  Below
+// ~2^11 elements a round is latency-bound (a short chain of products per lane
+// and the transcript step), so one workgroup is as fast as a grid and saves
+// two launches per round.
+constexpr int kTailThreads = 512;
+constexpr uint64_t kTailElems = 4 * (uint64_t)kTailThreads;
+
+__global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, fr_t* b, fr_t* c,
+                                                          fr_t* eq, uint64_t cur, int idx0) {
+  __shared__ fr_t alpha_sh;
+  __shared__ fr_t wsum[3][kTailThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  fr_t* mle[4] = {a, b, c, eq};
+  if (t == 0) alpha_sh = st->alpha[idx0 - 1];
+  __syncthreads();
+  const int m = st->m;
+  for (int idx = idx0; idx < m; idx++, cur >>= 1) {
+    const uint64_t h = cur / 4;
+    const fr_t f = alpha_sh;
+    fr_t v[3] = {fr_zero(), fr_zero(), fr_zero()};
+    if ((uint64_t)t < h) {
+      const uint64_t i = t;
+      fr_t lo[4], hi[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        fr_t x0 = fr_load(mle[j] + i), x1 = fr_load(mle[j] + i + h);
+        const fr_t x2 = fr_load(mle[j] + i + 2 * h), x3 = fr_load(mle[j] + i + 3 * h);
+        x0 = fr_add(x0, fr_mul(f, fr_sub(x2, x0)));
+        x1 = fr_add(x1, fr_mul(f, fr_sub(x3, x1)));
+        fr_t w0 = x0, w1 = x1;
+        fr_canon(w0);
+        fr_canon(w1);
+        fr_store(mle[j] + i, w0);
+        fr_store(mle[j] + i + h, w1);
+        lo[j] = x0;
+        hi[j] = x1;
+      }
+      v[0] = r1cs_term(lo[3], lo[0], lo[1], lo[2]);
+      const fr_t ea = fr_sub(fr_add(lo[3], lo[3]), hi[3]);
+      const fr_t aa = fr_sub(fr_add(lo[0], lo[0]), hi[0]);
+      const fr_t bb = fr_sub(fr_add(lo[1], lo[1]), hi[1]);
+      const fr_t cc = fr_sub(fr_add(lo[2], lo[2]), hi[2]);
+      v[1] = fr_mul(ea, fr_sub(fr_mul(aa, bb), cc));
+      v[2] = fr_mul(fr_mul(fr_sub(hi[3], lo[3]), fr_sub(hi[0], lo[0])), fr_sub(hi[1], lo[1]));
+    }
+    // wave sums by shuffles, then the per-wave sums on thread 0
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) v[q] = fr_add(v[q], fr_shfl(v[q], lane ^ w));
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) wsum[q][wave] = v[q];
+    }
+    __syncthreads();
+    fr_t h3[3] = {fr_zero(), fr_zero(), fr_zero()};
+    if (t == 0) {
+      for (int w = 0; w < kTailThreads / 64; w++) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) h3[q] = fr_add(h3[q], wsum[q][w]);
+      }
+#pragma unroll
+      for (int q = 0; q < 3; q++) fr_canon(h3[q]);
+    }
+    const fr_t al = finalize_round(st, idx, h3[0], h3[1], h3[2], nullptr);
+    if (t == 0) alpha_sh = al;
+    __syncthreads();
   }
 }
 
@@ -448,12 +576,12 @@ void sc_round_dev(fr_t* a, fr_t* b, fr_t* c, fr_t* eq, uint64_t size, const fr_t
                   fr_t* partials, fr_t* out3, hipStream_t s) {
   const uint64_t h = fold ? size / 4 : size / 2;
   const unsigned nb = round_blocks(h);
-  k_sc_round<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, fold, partials);
+  k_sc_round<false><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, fold, partials);
   k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, out3);
   PK_HIP(hipGetLastError());
 }
 
-size_t sc_partials_elems() { return 3 * (size_t)kMaxRoundBlocks; }
+size_t sc_partials_elems() { return 4 * (size_t)kMaxRoundBlocks; }
 
 void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int mode,
                   uint64_t seed, fr_t* work, hipStream_t s) {
@@ -464,22 +592,36 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
   PK_HIP(hipGetLastError());
   eq_table_dev(st->r, m, 1ull << m, eq, s);
   if (m == 0) zero_dev(eq + 1, 1, s);
-  if (mode == 0) {
+  // The plain mode's claim comes with round 0's partials (k_sc_round<true>);
+  // with no rounds (m = 0) it takes its own pass.  The reference mode's
+  // claim is rho * sum_g, from the blinding polynomial alone.
+  if (mode == 0 && m == 0) {
     const unsigned nb = round_blocks(size);
     k_sc_claim<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, partials);
     k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, h3);
   }
-  k_sc_claim_finalize<<<1, 64, 0, s>>>(st, h3);
+  if (mode != 0 || m == 0) k_sc_claim_finalize<<<1, 64, 0, s>>>(st, h3);
   PK_HIP(hipGetLastError());
   uint64_t cur = size;
   for (int idx = 0; idx < m; idx++) {
     const fr_t* fold = idx ? &st->alpha[idx - 1] : nullptr;
+    if (fold && cur <= kTailElems) {
+      // the remaining rounds in one workgroup
+      k_sc_tail<<<1, kTailThreads, 0, s>>>(st, a, b, c, eq, cur, idx);
+      PK_HIP(hipGetLastError());
+      break;
+    }
     // two launches per round: fold + map + workgroup sums, then the
     // one-wave sum of the partials and transcript step
     const unsigned nb = round_blocks(fold ? cur / 4 : cur / 2);
-    k_sc_round<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
+    if (idx == 0 && mode == 0) {
+      k_sc_round<true><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
+      k_sc_finalize<4><<<1, 64, 0, s>>>(st, partials, nb, idx);
+    } else {
+      k_sc_round<false><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
+      k_sc_finalize<3><<<1, 64, 0, s>>>(st, partials, nb, idx);
+    }
     if (fold) cur /= 2;
-    k_sc_finalize<<<1, 64, 0, s>>>(st, partials, nb, idx);
     PK_HIP(hipGetLastError());
   }
 }

@@ -277,8 +277,9 @@ def test_gpu_round(gpu, oracle, m):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("m,mode", [(1, "plain"), (2, "plain"), (5, "plain"), (10, "plain"),
-                                    (16, "plain"), (5, "ref"), (12, "ref")])
+@pytest.mark.parametrize("m,mode", [(1, "plain"), (2, "plain"), (3, "plain"), (5, "plain"),
+                                    (10, "plain"), (11, "plain"), (12, "plain"), (16, "plain"),
+                                    (1, "ref"), (3, "ref"), (5, "ref"), (11, "ref"), (12, "ref")])
 def test_gpu_sumcheck_prove(gpu, oracle, m, mode):
     n = 1 << m
     a = random_fr(n, seed=m, stream="sa")
