@@ -35,23 +35,39 @@ __device__ __forceinline__ fr_t fr_zero() {
 }
 
 // ------------------------------------------------------------------ SpMV
-// Device-resident matrices are not validated on the host, so the kernel
-// keeps every access in bounds whatever the indices: row ranges are clipped
+// Device-resident matrices are not validated on the host, so the kernels
+// keep every access in bounds whatever the indices: row ranges are clipped
 // to [0, nnz] and entries with a column >= ncols are skipped (a malformed
 // matrix gives an unspecified result, never an out-of-bounds read).
-__global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
-                                              const uint32_t* row_starts, uint64_t nnz,
-                                              const uint32_t* cols, const fr_t* vals,
-                                              const fr_t* z, fr_t* out) {
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const uint64_t e = (r + 1 < rows) ? min((uint64_t)row_starts[r + 1], nnz) : nnz;
-  const uint64_t b = min((uint64_t)row_starts[r], e);
+//
+// Rows of at most kShortRow entries (R1CS rows hold a handful) take one lane
+// each.  A longer row on one lane would be a straggler (a transposed matrix
+// has a row per witness column, and the constant-one column can appear in
+// most constraints), so long rows are summed by entry chunks instead: the
+// row's lane flags the kChunk-entry chunks it covers, one workgroup per
+// flagged chunk sums every long row's entries inside the chunk (rows that
+// start and end inside it are finished there; a row crossing the chunk's
+// first entry leaves a head partial, one crossing its last entry a tail
+// partial), and a last pass adds the partials of each row that spans
+// chunks.  The lanes of long rows list the spanning ones, so no host
+// round-trip is needed.
+constexpr uint64_t kShortRow = 32;
+constexpr uint64_t kChunk = 2048;
+constexpr int kMaxChunkRows = (int)(kChunk / (kShortRow + 1)) + 2;  // long rows meeting one chunk
+
+__device__ __forceinline__ void row_range(const uint32_t* row_starts, uint64_t rows, uint64_t nnz,
+                                          uint64_t r, uint64_t& b, uint64_t& e) {
+  e = (r + 1 < rows) ? min((uint64_t)row_starts[r + 1], nnz) : nnz;
+  b = min((uint64_t)row_starts[r], e);
+}
+
+// sum of v[i] z[col[i]] over [b, e), one lane
+__device__ __forceinline__ fr_t row_dot(uint64_t b, uint64_t e, uint64_t ncols, const uint32_t* cols,
+                                        const fr_t* vals, const fr_t* z) {
   fr_t acc = fr_zero();
-  // R1CS rows hold a handful of entries: take them four at a time, with all
-  // index, value and gather loads issued before the products, so a lane
-  // keeps several independent HBM requests in flight instead of one chain
-  // (index -> gather -> product) per entry
+  // take the entries four at a time, with all index, value and gather loads
+  // issued before the products, so a lane keeps several independent HBM
+  // requests in flight instead of one chain (index -> gather -> product)
   constexpr int G = 4;
   for (uint64_t i0 = b; i0 < e; i0 += G) {
     uint32_t c[G];
@@ -69,8 +85,142 @@ __global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
     for (int k = 0; k < G; k++)
       if (c[k] < ncols) acc = fr_add(acc, fr_mul(v[k], x[k]));
   }
+  return acc;
+}
+
+// spans: [0] = count, then the spanning long rows (capacity n_chunks)
+__global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
+                                              const uint32_t* row_starts, uint64_t nnz,
+                                              const uint32_t* cols, const fr_t* vals,
+                                              const fr_t* z, fr_t* out, uint32_t* chunk_flags,
+                                              uint32_t* spans, uint64_t n_chunks) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  uint64_t b, e;
+  row_range(row_starts, rows, nnz, r, b, e);
+  if (e - b > kShortRow) {
+    const uint64_t c0 = b / kChunk, c1 = (e - 1) / kChunk;
+    for (uint64_t c = c0; c <= c1 && c < n_chunks; c++) chunk_flags[c] = 1u;
+    if (c1 > c0) {
+      const uint32_t slot = atomicAdd(spans, 1u);
+      if (slot < n_chunks) spans[1 + slot] = (uint32_t)r;
+    }
+    return;
+  }
+  fr_t acc = row_dot(b, e, ncols, cols, vals, z);
   fr_canon(acc);
   fr_store(out + r, acc);
+}
+
+// Workgroup sum (256 threads) of one value per thread; valid on thread 0.
+__device__ __forceinline__ fr_t block_sum1(fr_t v) {
+  __shared__ fr_t part[4];
+  const int t = threadIdx.x, lane = t & 63;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) v = fr_add(v, fr_shfl(v, lane ^ w));
+  __syncthreads();  // part may still be read from a previous call
+  if (lane == 0) part[t >> 6] = v;
+  __syncthreads();
+  fr_t s = fr_zero();
+  if (t == 0) s = fr_add(fr_add(part[0], part[1]), fr_add(part[2], part[3]));
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_spmv_chunks(uint64_t rows, uint64_t ncols,
+                                                     const uint32_t* row_starts, uint64_t nnz,
+                                                     const uint32_t* cols, const fr_t* vals,
+                                                     const fr_t* z, fr_t* out,
+                                                     const uint32_t* chunk_flags, fr_t* head,
+                                                     fr_t* tail, uint64_t n_chunks) {
+  __shared__ uint64_t r_lo;
+  __shared__ uint32_t list[kMaxChunkRows];
+  __shared__ int nlist;
+  const int t = threadIdx.x;
+  // a grid of at most 1024 workgroups strides over the chunks: most chunks
+  // of most matrices have no long row, and a workgroup per chunk would cost
+  // more to launch than to skip
+  for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    if (!chunk_flags[c]) continue;
+    const uint64_t c0 = c * kChunk, c1 = min(c0 + kChunk, nnz);
+    if (t == 0) {
+      // the last row starting at or before c0 (0 if none)
+      uint64_t lo = 0, hi = rows;
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (row_starts[mid] <= c0)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      r_lo = lo;
+      nlist = 0;
+    }
+    __syncthreads();
+    for (uint64_t rr = r_lo + t; rr < rows; rr += blockDim.x) {
+      uint64_t b, e;
+      row_range(row_starts, rows, nnz, rr, b, e);
+      if (b >= c1) break;
+      if (e > c0 && e - b > kShortRow) {
+        const int i = atomicAdd(&nlist, 1);
+        if (i < kMaxChunkRows) list[i] = (uint32_t)rr;
+      }
+    }
+    __syncthreads();
+    const int n = min(nlist, kMaxChunkRows);
+    for (int k = 0; k < n; k++) {
+      const uint64_t rr = list[k];
+      uint64_t b, e;
+      row_range(row_starts, rows, nnz, rr, b, e);
+      const uint64_t lo = max(b, c0), hi = min(e, c1);
+      fr_t acc = fr_zero();
+      for (uint64_t i = lo + t; i < hi; i += blockDim.x) {
+        const uint32_t cc = cols[i];
+        if (cc < ncols) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + cc)));
+      }
+      fr_t sum = block_sum1(acc);
+      if (t == 0) {
+        fr_canon(sum);
+        if (b >= c0 && e <= c1)
+          fr_store(out + rr, sum);   // the whole row is in this chunk
+        else if (b < c0)
+          fr_store(head + c, sum);   // continues from earlier chunks
+        else
+          fr_store(tail + c, sum);   // starts here, continues after
+      }
+    }
+    __syncthreads();  // r_lo, list and nlist are rewritten for the next chunk
+  }
+}
+
+// Rows spanning chunks: the tail partial of the first chunk plus the head
+// partials of the others.
+__global__ __launch_bounds__(256) void k_spmv_spans(uint64_t rows, const uint32_t* row_starts,
+                                                    uint64_t nnz, const uint32_t* spans,
+                                                    uint64_t n_chunks, const fr_t* head,
+                                                    const fr_t* tail, fr_t* out) {
+  const uint64_t count = min((uint64_t)spans[0], n_chunks);
+  const int t = threadIdx.x;
+  for (uint64_t j = blockIdx.x; j < count; j += gridDim.x) {
+    const uint64_t rr = min((uint64_t)spans[1 + j], rows - 1);
+    uint64_t b, e;
+    row_range(row_starts, rows, nnz, rr, b, e);
+    fr_t acc = fr_zero();
+    if (e > b) {
+      const uint64_t cf = b / kChunk, cl = min((e - 1) / kChunk, n_chunks - 1);
+      for (uint64_t cc = cf + 1 + t; cc <= cl; cc += blockDim.x) acc = fr_add(acc, fr_load(head + cc));
+      if (t == 0 && cf < n_chunks) acc = fr_add(acc, fr_load(tail + cf));
+    }
+    fr_t sum = block_sum1(acc);
+    if (t == 0) {
+      fr_canon(sum);
+      fr_store(out + rr, sum);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_clear(uint32_t* flags, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = 0u;
 }
 
 // c[i] = a[i] * b[i] for i < n, zero up to n_pad
@@ -529,8 +679,19 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
 // ------------------------------------------------------------------ launchers
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {
   if (!A.rows) return;
+  const uint64_t n_chunks = A.nnz ? (A.nnz + kChunk - 1) / kChunk : 1;
+  // flags[n_chunks], spans[1 + n_chunks], head/tail partials[n_chunks]
+  Scratch flags(4 * (2 * n_chunks + 1), s), part(2 * sizeof(fr_t) * n_chunks, s);
+  uint32_t* fl = flags.as<uint32_t>();
+  uint32_t* spans = fl + n_chunks;
+  fr_t* head = part.as<fr_t>();
+  fr_t* tail = head + n_chunks;
+  k_spmv_clear<<<nblk(n_chunks + 1, 256), 256, 0, s>>>(fl, n_chunks + 1);  // flags and the count
   k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals,
-                                            z, out);
+                                            z, out, fl, spans, n_chunks);
+  k_spmv_chunks<<<(unsigned)std::min<uint64_t>(n_chunks, 1024), 256, 0, s>>>(
+      A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals, z, out, fl, head, tail, n_chunks);
+  k_spmv_spans<<<64, 256, 0, s>>>(A.rows, A.row_starts, A.nnz, spans, n_chunks, head, tail, out);
   PK_HIP(hipGetLastError());
 }
 

@@ -185,6 +185,70 @@ def test_gpu_sparse_mul(gpu, oracle, log_rows):
     assert np.array_equal(A * z, oracle.sparse_mul(rows, cols, rs, cs, vs, z))
 
 
+def long_row_cases():
+    """Row-length shapes around the device's per-lane limit (32 entries) and
+    its 2048-entry chunks: (name, rows, cols, row lengths)."""
+    return [
+        ("just-long", 40, 100, [33, 1, 32, 34] + [2] * 36),
+        ("one-huge", 10, 30000, [1, 2, 25000, 3, 0, 0, 4, 1, 1, 1]),
+        ("chunk-aligned", 6, 9000, [2048, 2048, 4096, 1, 33, 2047]),
+        ("adjacent-long", 8, 5000, [1500, 1500, 1500, 40, 1, 3000, 0, 60]),
+        ("many-long-in-chunk", 70, 200, [33] * 70),
+        ("long-after-empty", 7, 4000, [0, 0, 0, 3500, 0, 2, 0]),
+    ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", long_row_cases(), ids=lambda c: c[0])
+def test_gpu_sparse_mul_long_rows(gpu, oracle, case):
+    name, rows, cols, lens = case
+    rng = np.random.default_rng(len(name))
+    rs, cs = [], []
+    for k in lens:
+        rs.append(len(cs))
+        cs.extend(sorted(rng.choice(cols, size=k, replace=False).tolist()))
+    rs, cs = np.array(rs, np.uint32), np.array(cs, np.uint32)
+    vs = random_fr(len(cs), seed=3, stream="long-rows") if len(cs) else np.zeros((0, 4), np.uint64)
+    z = random_fr(cols, seed=6, stream="long-rows-z")
+    print(f"[spmv] {name} rows={rows} nnz={len(cs)}", flush=True)
+    A = gpu.SparseMatrix(rows, cols, rs, cs, vs)
+    assert np.array_equal(A * z, oracle.sparse_mul(rows, cols, rs, cs, vs, z))
+
+
+@pytest.mark.gpu
+def test_gpu_sparse_mul_constant_column_transposed(gpu, oracle):
+    # a transposed R1CS matrix whose first row (the constant-one column)
+    # holds an entry for most constraints: 2^18 rows, one row of ~2^17 entries
+    n = 1 << 18
+    rng = np.random.default_rng(21)
+    rs, cs, vs = random_csr(rng, n, n, 21)
+    rows_with_one = np.sort(rng.choice(n, size=n // 2, replace=False))
+    # add column 0 to half the rows (keeping columns sorted and unique)
+    new_rs, new_cs, new_vs = [], [], []
+    ends = np.append(rs[1:], len(cs))
+    mark = np.zeros(n, bool)
+    mark[rows_with_one] = True
+    extra = random_fr(n // 2, seed=8, stream="const-col")
+    ei = 0
+    for r in range(n):
+        new_rs.append(len(new_cs))
+        seg_c, seg_v = cs[rs[r]:ends[r]], vs[rs[r]:ends[r]]
+        if mark[r] and (len(seg_c) == 0 or seg_c[0] != 0):
+            new_cs.append(0)
+            new_vs.append(extra[ei])
+            ei += 1
+        new_cs.extend(seg_c.tolist())
+        new_vs.extend(seg_v)
+    rs2, cs2 = np.array(new_rs, np.uint32), np.array(new_cs, np.uint32)
+    vs2 = np.array(new_vs, np.uint64).reshape(-1, 4)
+    t = oracle.sparse_transpose(n, n, rs2, cs2, vs2)
+    z = random_fr(n, seed=9, stream="const-col-z")
+    print(f"[spmv] transposed constant column: longest row {int(np.diff(np.append(t[0], len(t[1]))).max())}",
+          flush=True)
+    At = gpu.SparseMatrix(n, n, *t)
+    assert np.array_equal(At * z, oracle.sparse_mul(n, n, *t, z))
+
+
 @pytest.mark.gpu
 def test_gpu_sparse_mul_device_malformed_stays_in_bounds(gpu, oracle):
     # device-resident matrices are not validated: row ranges are clipped to
