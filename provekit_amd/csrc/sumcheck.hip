@@ -508,15 +508,20 @@ __device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t&
       stt = sky_compress(stt, cl);
     }
     state = stt;
-    fr_t g[4] = {fr_zero(), fr_zero(), fr_zero(), fr_zero()};
-    if (st->mode == 1) blinding_coeffs(st, idx, g);
-    const fr_t S = st->S, rho = st->rho;
-    fr_t c0 = fr_add(h0, fr_mul(rho, g[0]));
-    const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
-    const fr_t at_m1 = fr_add(hm1, fr_mul(rho, gm1));
+    const fr_t S = st->S;
+    fr_t c0 = h0, at_m1 = hm1, c3 = hinf;
+    if (st->mode == 1) {
+      // the blinding polynomial's part, rho * g (plain mode: rho = 0)
+      fr_t g[4];
+      blinding_coeffs(st, idx, g);
+      const fr_t rho = st->rho;
+      const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
+      c0 = fr_add(c0, fr_mul(rho, g[0]));
+      at_m1 = fr_add(at_m1, fr_mul(rho, gm1));
+      c3 = fr_add(c3, fr_mul(rho, g[3]));
+    }
     const fr_t three_c0 = fr_add(fr_add(c0, c0), c0);
     fr_t c2 = fr_mul(st->half, fr_sub(fr_add(S, at_m1), three_c0));
-    fr_t c3 = fr_add(hinf, fr_mul(rho, g[3]));
     fr_t c1 = fr_sub(fr_sub(fr_sub(fr_sub(S, c0), c0), c3), c2);
     fr_canon(c0);
     fr_canon(c1);
