@@ -132,6 +132,17 @@ int pkgpu_skyscraper_hash_many_device(size_t size, const void* input, size_t cou
  * last (nodes may equal leaves). */
 int pkgpu_skyscraper_merkle_device(const void* leaves, size_t n, void* nodes, void* stream);
 
+/* Openings from a device-resident commitment (the codeword and nodes of
+ * pkgpu_commit_device, or a Poseidon2 tree with the same node layout): for
+ * each of the n host leaf indices (< leaves, a power of two), values (host)
+ * receives the leaf's leaf_elems elements and paths (host) the log2(leaves)
+ * sibling digests of 32 bytes, leaf level first.  The data a whir query
+ * opening reads (the protocol itself lives in the unvendored whir crate);
+ * the tree is not copied back. */
+int pkgpu_merkle_open_device(const uint64_t* codeword, const void* nodes, size_t leaves,
+                             size_t leaf_elems, const uint64_t* leaf_indices, size_t n,
+                             uint64_t* values, uint8_t* paths, void* stream);
+
 /* ------------------------------------------------------ proof of work */
 /* skyscraper::pow::threshold (skyscraper/core/src/pow.rs:18-26): 4 LE u64.
  * PKGPU_ERR_ARG ("Difficulty must be in the range [0, 80)") outside [0, 80),

@@ -65,6 +65,7 @@ SIGNATURES = {
     "pkgpu_skyscraper_hash_many": [_sz, _vp, _sz, _vp, _sz],
     "pkgpu_skyscraper_hash_many_device": [_sz, _vp, _sz, _vp, _vp],
     "pkgpu_skyscraper_merkle_device": [_vp, _sz, _vp, _vp],
+    "pkgpu_merkle_open_device": [_vp, _vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp],
     "pkgpu_skyscraper_pow_threshold": [ctypes.c_double, _vp],
     "pkgpu_skyscraper_pow_verify": [_vp, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_i)],
     "pkgpu_skyscraper_pow_solve": [_vp, ctypes.c_double, ctypes.c_uint64,
@@ -700,6 +701,20 @@ def hash_many_device(size: int, inp: DeviceBuffer, count: int, out: DeviceBuffer
 
 def merkle_device(leaves: DeviceBuffer, n: int, nodes: DeviceBuffer, stream=None) -> None:
     _call("pkgpu_skyscraper_merkle_device", leaves.ptr, n, nodes.ptr, stream)
+
+
+def merkle_open_device(codeword: DeviceBuffer, nodes: DeviceBuffer, leaves: int, leaf_elems: int,
+                       leaf_indices, stream=None):
+    """Openings from a resident tree: (values (n, leaf_elems, 4), paths (n, depth, 32 bytes))."""
+    idx = np.ascontiguousarray(leaf_indices, dtype=np.uint64)
+    n = idx.shape[0]
+    depth = max(leaves.bit_length() - 1, 0)
+    values = np.zeros((n, leaf_elems, 4), np.uint64)
+    paths = np.zeros((n, depth, 32), np.uint8)
+    _call("pkgpu_merkle_open_device", codeword.ptr, nodes.ptr, leaves, leaf_elems,
+          _ptr(idx) if n else None, n, _ptr(values) if n else None,
+          _ptr(paths) if paths.size else None, stream)
+    return values, paths
 
 
 class BatchCommit:

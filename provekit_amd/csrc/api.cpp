@@ -512,6 +512,31 @@ int pkgpu_skyscraper_pow_solve(const uint64_t challenge[4], double difficulty,
   });
 }
 
+int pkgpu_merkle_open_device(const uint64_t* codeword, const void* nodes, size_t leaves,
+                             size_t leaf_elems, const uint64_t* leaf_indices, size_t n,
+                             uint64_t* values, uint8_t* paths, void* stream) {
+  return guarded([&] {
+    if (!n) return;
+    REQ(codeword && nodes && leaf_indices && values && (leaves < 2 || paths), pk::PK_ERR_ARG,
+        "merkle open: null pointer");
+    REQ(pk::is_pow2(leaves), pk::PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
+    REQ(leaf_elems > 0, pk::PK_ERR_SIZE, "merkle open: empty leaves");
+    for (size_t q = 0; q < n; q++)
+      REQ(leaf_indices[q] < leaves, pk::PK_ERR_ARG, "merkle open: leaf index out of range");
+    hipStream_t s = stream_or_thread(stream);
+    const int depth = pk::log2_exact(leaves);
+    Staged di(8 * n, s), dv(sizeof(fr_t) * n * leaf_elems, s),
+        dp(sizeof(fr_t) * n * (depth ? depth : 1), s);
+    h2d(di.get<void>(), leaf_indices, 8 * n, s);
+    pk::merkle_open_dev(reinterpret_cast<const fr_t*>(codeword), static_cast<const fr_t*>(nodes),
+                        leaves, leaf_elems, di.get<uint64_t>(), n, dv.get<fr_t>(), dp.get<fr_t>(),
+                        s);
+    d2h(values, dv.get<void>(), sizeof(fr_t) * n * leaf_elems, s);
+    if (depth) d2h(paths, dp.get<void>(), sizeof(fr_t) * n * depth, s);
+    sync(s);
+  });
+}
+
 int pkgpu_poseidon2_permutation_many(const uint64_t* states, size_t n, uint64_t* out) {
   return guarded([&] {
     if (!n) return;

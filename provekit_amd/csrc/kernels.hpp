@@ -73,6 +73,12 @@ void sky_hash_many_dev(uint64_t chunks, const fr_t* in, uint64_t count, fr_t* ou
 // Binary tree over n (power of two) leaf digests: nodes[0..n) = leaves (may
 // alias `leaves`), levels follow, root at nodes[2n-2].
 void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s);
+// Openings from a resident tree (any 2-to-1 tree in this node layout): for
+// query q with leaf index idx[q] < leaves (device array), values gets the
+// leaf's leaf_elems codeword elements and paths the log2(leaves) sibling
+// digests, leaf level first.
+void merkle_open_dev(const fr_t* codeword, const fr_t* nodes, uint64_t leaves, uint64_t leaf_elems,
+                     const uint64_t* idx, uint64_t n, fr_t* values, fr_t* paths, hipStream_t s);
 
 // ---- proof of work (pow.hip) ----
 // f64_to_u256 and threshold of skyscraper/core/src/pow.rs:18-26,61-88

@@ -27,21 +27,6 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 
 uint64_t align_up(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
-// Leaf opening: q-th query leaf index i -> leaf elements and the sibling
-// digest at every level (level-by-level node layout, root last).
-__global__ void k_open(const fr_t* codeword, const fr_t* nodes, uint64_t leaves, uint64_t leaf_elems,
-                       int depth, const uint64_t* idx, uint64_t n, fr_t* values, fr_t* paths) {
-  const uint64_t q = blockIdx.x;
-  if (q >= n) return;
-  const uint64_t i = idx[q];
-  for (uint64_t k = threadIdx.x; k < leaf_elems; k += blockDim.x)
-    fr_store(values + q * leaf_elems + k, fr_load(codeword + i * leaf_elems + k));
-  for (int l = threadIdx.x; l < depth; l += blockDim.x) {
-    const uint64_t off = 2 * leaves - 2 * (leaves >> l);
-    fr_store(paths + q * depth + l, fr_load(nodes + off + (((i >> l) ^ 1))));
-  }
-}
-
 }  // namespace
 
 struct Server::Job {
@@ -350,9 +335,8 @@ void Server::open(uint64_t id, const uint64_t* leaves_idx, size_t n, fr_t* value
   {
     Scratch di(8 * n, s), dv(32 * n * leaf_elems, s), dp(32 * n * (depth ? depth : 1), s);
     PK_HIP(hipMemcpyAsync(di.p, leaves_idx, 8 * n, hipMemcpyHostToDevice, s));
-    k_open<<<(unsigned)n, 64, 0, s>>>(codeword, nodes, leaves, leaf_elems, depth,
-                                       di.as<uint64_t>(), n, dv.as<fr_t>(), dp.as<fr_t>());
-    PK_HIP(hipGetLastError());
+    merkle_open_dev(codeword, nodes, leaves, leaf_elems, di.as<uint64_t>(), n, dv.as<fr_t>(),
+                    dp.as<fr_t>(), s);
     PK_HIP(hipMemcpyAsync(values, dv.p, 32 * n * leaf_elems, hipMemcpyDeviceToHost, s));
     if (depth) PK_HIP(hipMemcpyAsync(paths, dp.p, 32 * n * depth, hipMemcpyDeviceToHost, s));
   }

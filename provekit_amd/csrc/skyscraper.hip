@@ -65,6 +65,22 @@ __global__ void k_copy(const fr_t* in, fr_t* out, uint64_t n) {
   if (i < n) fr_store(out + i, fr_load(in + i));
 }
 
+// Leaf opening: query q's leaf index i -> the leaf's elements and the
+// sibling digest at every level (level-by-level node layout, root last).
+__global__ void k_open(const fr_t* codeword, const fr_t* nodes, uint64_t leaves, uint64_t leaf_elems,
+                       int depth, const uint64_t* idx, uint64_t n, fr_t* values, fr_t* paths) {
+  const uint64_t q = blockIdx.x;
+  if (q >= n) return;
+  const uint64_t i = idx[q];
+  if (i >= leaves) return;  // the host checks; never read outside the tree
+  for (uint64_t k = threadIdx.x; k < leaf_elems; k += blockDim.x)
+    fr_store(values + q * leaf_elems + k, fr_load(codeword + i * leaf_elems + k));
+  for (int l = threadIdx.x; l < depth; l += blockDim.x) {
+    const uint64_t off = 2 * leaves - 2 * (leaves >> l);
+    fr_store(paths + q * depth + l, fr_load(nodes + off + (((i >> l) ^ 1))));
+  }
+}
+
 inline unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -105,6 +121,16 @@ void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s) 
     k_merkle_tail<<<1, 64, 0, s>>>(lvl, n);
     PK_HIP(hipGetLastError());
   }
+}
+
+void merkle_open_dev(const fr_t* codeword, const fr_t* nodes, uint64_t leaves, uint64_t leaf_elems,
+                     const uint64_t* idx, uint64_t n, fr_t* values, fr_t* paths, hipStream_t s) {
+  if (!n) return;
+  PK_REQUIRE(is_pow2(leaves), PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
+  PK_REQUIRE(n < (1ull << 31), PK_ERR_SIZE, "merkle open: too many queries");
+  k_open<<<(unsigned)n, 64, 0, s>>>(codeword, nodes, leaves, leaf_elems, log2_exact(leaves), idx, n,
+                                     values, paths);
+  PK_HIP(hipGetLastError());
 }
 
 }  // namespace pk

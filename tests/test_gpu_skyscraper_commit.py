@@ -107,3 +107,53 @@ def test_commit_2p20_blowup4_full_size(gpu, oracle):
     assert np.array_equal(nodes[: 1 << 18], enodes[: 1 << 18]), "leaf digests"
     assert np.array_equal(nodes, enodes), "tree"
     assert root == eroot
+
+
+@pytest.mark.parametrize("log_n,blowup,leaf", [(3, 2, 16), (10, 4, 16), (12, 2, 4), (14, 4, 16)])
+def test_merkle_open_from_resident_commit(gpu, oracle, log_n, blowup, leaf):
+    # the tree stays on the device; openings come back against the oracle's
+    # codeword and nodes, and each path hashes back to the root
+    n = 1 << log_n
+    c = random_fr(n, seed=300 + log_n, stream="open")
+    N = n * blowup
+    leaves = N // leaf
+    dc = gpu.DeviceBuffer.from_array(c)
+    dcw, dnodes = gpu.DeviceBuffer(32 * N), gpu.DeviceBuffer(32 * (2 * leaves - 1))
+    say(f"open from resident commit 2^{log_n} x{blowup} leaf={leaf}")
+    gpu.commit_device(dc, n, blowup, True, leaf, dcw, dnodes)
+    gpu.synchronize()
+    root, ecw, enodes = oracle.commit(c, blowup, True, leaf)
+    rng = np.random.default_rng(log_n)
+    idx = np.concatenate([[0, leaves - 1], rng.integers(0, leaves, size=64)]).astype(np.uint64)
+    values, paths = gpu.merkle_open_device(dcw, dnodes, leaves, leaf, idx)
+    depth = leaves.bit_length() - 1
+    for q, i in enumerate(idx.tolist()):
+        assert np.array_equal(values[q], ecw[i * leaf:(i + 1) * leaf])
+        # Merkle verification from the definition: leaf digest, then siblings
+        h = np.frombuffer(enodes[i].tobytes(), np.uint64)
+        for l in range(depth):
+            sib = np.frombuffer(paths[q, l].tobytes(), np.uint64)
+            h = oracle.sky_compress(sib, h) if (i >> l) & 1 else oracle.sky_compress(h, sib)
+        assert h.tobytes() == root
+    with pytest.raises(gpu.ProvekitGpuError):
+        gpu.merkle_open_device(dcw, dnodes, leaves, leaf, [leaves])
+
+
+def test_merkle_open_poseidon2_tree(gpu, oracle):
+    # a Poseidon2 tree has the same node layout: leaves of one element
+    n = 1 << 12
+    x = random_fr(n, seed=41, stream="p2-open")
+    dx = gpu.DeviceBuffer.from_array(x)
+    dn = gpu.DeviceBuffer(32 * (2 * n - 1))
+    g = gpu.poseidon2_merkle_graph(dx, n, dn)
+    g.launch()
+    gpu.synchronize()
+    enodes = oracle.p2_merkle_levels(x)
+    idx = np.array([0, 1, 2, 777, n - 1], np.uint64)
+    values, paths = gpu.merkle_open_device(dx, dn, n, 1, idx)
+    for q, i in enumerate(idx.tolist()):
+        assert np.array_equal(values[q, 0], x[i])
+        for l in range(12):
+            off = 2 * n - 2 * (n >> l)
+            assert paths[q, l].tobytes() == enodes[off + ((i >> l) ^ 1)].tobytes()
+    g.destroy()
