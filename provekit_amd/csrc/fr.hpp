@@ -147,56 +147,32 @@ __device__ __forceinline__ fr_t fr_sub_canon(const fr_t& a, const fr_t& b) {
 }
 
 // ---------------------------------------------------------------- Montgomery
-// One column step of product scanning: (c2:c01) += a*b.  The 64-bit add of the
-// product into the aligned accumulator pair carries out into an SGPR lane
-// mask, which the following add-with-carry folds into the third word.  Plain C
-// does not get the compiler to use the carry-out of v_mad_u64_u32, and 64-bit
-// operands must sit in even-aligned register pairs, so a CIOS row written in C
-// spends about three moves per digit product; this pair spends none.
-// ALU only: no memory operand, nothing left in flight.
-__device__ __forceinline__ void macc(uint64_t& c01, uint32_t& c2, uint32_t a, uint32_t b) {
-  uint64_t cy;
-  asm("v_mad_u64_u32 %0, %2, %3, %4, %0\n\t"
-      "v_addc_co_u32_e64 %1, %2, 0, %1, %2"
-      : "+v"(c01), "+v"(c2), "=&s"(cy)
-      : "v"(a), "v"(b));
-}
-
+// Each digit product is a multiply-add pair: v_mad_u64_u32 adds the 32x32
+// product into the aligned 64-bit accumulator pair and carries out into an
+// SGPR lane mask, which the following v_addc_co_u32 folds into the third
+// word.  Plain C does not get the compiler to use that carry-out, and 64-bit
+// operands must sit in even-aligned register pairs, so a row written in C
+// spends about three moves per digit product; the pair spends none.  ALU
+// only: no memory operand, nothing left in flight.
 // a, b lazy -> a*b*2^-256 lazy.
 // Montgomery multiplication in finely integrated product scanning (FIPS) form:
 // column k of a*b and of m*p are summed in one 96-bit accumulator; for k < 8
 // the quotient digit m_k is chosen so that the column's low word cancels, for
-// k >= 8 the low word is output digit k-8.
+// k >= 8 the low word is output digit k-8.  The body (fr_mul_asm.inc, written
+// by gen_fr_mul.py) holds one asm statement per column, not per multiply-add:
+// the compiler follows every asm statement with a wait state before its
+// outputs are used, which per multiply-add cost a lone wave ~18 % of its
+// product latency (1700 -> 1400 cycles measured; the same rate at full
+// occupancy).  The quotient digit -(x + (x << 28)) is asm too: written in C
+// the compiler folds it back into a v_mul_lo_u32.
 __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   uint32_t m[8];
   fr_t r;
   uint64_t c01 = 0;
   uint32_t c2 = 0;
-#pragma unroll
-  for (int k = 0; k < 15; k++) {
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int j = k - i;
-      if (j >= 0 && j < 8) macc(c01, c2, a.l[i], b.l[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int q = k - j;
-      if (j < k && q >= 1 && q < 8) macc(c01, c2, m[j], p_limb(q));
-    }
-    if (k < 8) {
-      // m = x * (-p^-1 mod 2^32) with -p^-1 = -(1 + 2^28): two full-rate ops
-      // instead of a quarter-rate v_mul_lo_u32
-      const uint32_t x = (uint32_t)c01;
-      m[k] = 0u - (x + (x << 28));
-      macc(c01, c2, m[k], PK_P0);
-    } else {
-      r.l[k - 8] = (uint32_t)c01;
-    }
-    c01 = (c01 >> 32) | ((uint64_t)c2 << 32);
-    c2 = 0;
-  }
-  r.l[7] = (uint32_t)c01;
+  uint64_t cy;
+#include "fr_mul_asm.inc"
+  (void)cy;
   return r;
 }
 

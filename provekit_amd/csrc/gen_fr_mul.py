@@ -1,0 +1,67 @@
+"""Writes fr_mul_asm.inc, the body of fr_mul in fr.hpp: the 8 x 32-bit
+Montgomery product with one inline-asm statement per column (run:
+python3 gen_fr_mul.py, from this folder).
+
+Why per column: the compiler cannot see into inline asm, so after every asm
+statement it inserts a wait state before the next use of the statement's
+outputs (the statement could hold an instruction with a hazard).  With one
+statement per multiply-add pair that was ~130 s_nop per product, and a lone
+wave (tree tops, transcript steps) spends a full issue slot on each: 1700
+cycles per product.  One statement per column, plus one per quotient digit,
+leaves ~23 and measures 1400 cycles; at full occupancy the rate is the same
+(the nops never held the VALU).  A single statement for the whole product
+(accumulator in fixed registers) measured the same latency and a slightly
+lower rate.
+
+Algorithm (finely integrated product scanning): column k of a*b and of m*p
+accumulates in the 96-bit (c2 : c01); for k < 8 the quotient digit
+m_k = lo * (-p^-1 mod 2^32) = -(lo + (lo << 28)) makes the low word cancel
+(two full-rate instructions: written in C the compiler folds them back into
+one v_mul_lo_u32), for k >= 8 the low word is output digit k - 8; then the
+accumulator shifts down one word (in C: register renaming and moves)."""
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def column(k):
+    terms = [(f"a.l[{i}]", f"b.l[{k - i}]", "v") for i in range(8) if 0 <= k - i < 8]
+    terms += [(f"m[{j}]", f"p_limb({k - j})", "s") for j in range(8) if j < k and 1 <= k - j < 8]
+    body, ops = [], []
+    n = 3  # %0 c01, %1 c2, %2 cy
+    for x, y, cls in terms:
+        body.append(f"v_mad_u64_u32 %0, %2, %{n}, %{n + 1}, %0")
+        body.append("v_addc_co_u32_e64 %1, %2, 0, %1, %2")
+        ops += [f'"v"({x})', f'"{cls}"({y})']
+        n += 2
+    out = []
+    if body:
+        text = "\\n\\t".join(body)
+        out.append(f'  asm("{text}"\n               : "+v"(c01), "+v"(c2), "=&s"(cy)\n'
+                   f'               : {", ".join(ops)});')
+    if k < 8:
+        # m_k from the column's low word, then m_k * p0 into the column
+        out.append(f'  asm("v_lshl_add_u32 %3, %4, 28, %4\\n\\tv_sub_u32 %3, 0, %3\\n\\t'
+                   f'v_mad_u64_u32 %0, %2, %3, %5, %0\\n\\tv_addc_co_u32_e64 %1, %2, 0, %1, %2"\n'
+                   f'               : "+v"(c01), "+v"(c2), "=&s"(cy), "=&v"(m[{k}])\n'
+                   f'               : "v"((uint32_t)c01), "s"(p_limb(0)));')
+    else:
+        out.append(f"  r.l[{k - 8}] = (uint32_t)c01;")
+    out.append("  c01 = (c01 >> 32) | ((uint64_t)c2 << 32);")
+    out.append("  c2 = 0;")
+    return out
+
+
+def main():
+    lines = ["// Generated by gen_fr_mul.py; do not edit.  The body of fr_mul (fr.hpp):",
+             "// in a, b (lazy); out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there."]
+    for k in range(15):
+        lines.append(f"  // column {k}")
+        lines += column(k)
+    lines.append("  r.l[7] = (uint32_t)c01;")
+    with open(os.path.join(HERE, "fr_mul_asm.inc"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+if __name__ == "__main__":
+    main()
