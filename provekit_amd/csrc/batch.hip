@@ -59,6 +59,9 @@ __device__ __forceinline__ uint64_t level_off(uint64_t n0, uint32_t l) {
 // the dispatcher places one wave per SIMD per workgroup (one-wave
 // workgroups can stack several waves on one SIMD while another idles).  No
 // workgroup barrier: broadcasts and the tail's level exchange are wave-level.
+// The kernel has three compression call sites (leaf fold, node, tail) and
+// every wave streams through them, so it takes the rolled compression: three
+// unrolled copies would not fit the instruction cache.
 constexpr int kQueueBlock = 256;
 
 __global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* polys,
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* 
       const uint64_t leaf = (uint64_t)tk.j * kQueueWave + t;
       if (leaf < p.leaves) {
         fr_store(p.nodes + leaf,
-                          sky_hash_message(p.codeword + leaf * p.leaf_elems, p.leaf_elems));
+                          sky_hash_message_t<false>(p.codeword + leaf * p.leaf_elems, p.leaf_elems));
       }
     } else if (tk.kind == kTaskNode) {
       const uint64_t nl = p.leaves >> tk.level;
@@ -90,8 +93,8 @@ __global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* 
       if (node < nl) {
         const fr_t* in = p.nodes + level_off(p.leaves, tk.level - 1);
         fr_store(p.nodes + level_off(p.leaves, tk.level) + node,
-                          sky_compress(fr_load(in + 2 * node),
-                                       fr_load(in + 2 * node + 1)));
+                          sky_compress_rolled(fr_load(in + 2 * node),
+                                              fr_load(in + 2 * node + 1)));
       }
     } else {  // tail: level tk.level (<= 64 nodes, one per lane) up to the root
       uint64_t n = p.leaves >> tk.level;
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* 
         const fr_t a = fr_shfl(cur, (2 * t) & (kQueueWave - 1));
         const fr_t b = fr_shfl(cur, (2 * t + 1) & (kQueueWave - 1));
         if ((uint64_t)t < h) {
-          cur = sky_compress(a, b);
+          cur = sky_compress_rolled(a, b);
           fr_store(p.nodes + off + t, cur);
         }
         off += h;

@@ -18,7 +18,10 @@ accumulates in the 96-bit (c2 : c01); for k < 8 the quotient digit
 m_k = lo * (-p^-1 mod 2^32) = -(lo + (lo << 28)) makes the low word cancel
 (two full-rate instructions: written in C the compiler folds them back into
 one v_mul_lo_u32), for k >= 8 the low word is output digit k - 8; then the
-accumulator shifts down one word (in C: register renaming and moves)."""
+accumulator shifts down one word (in C: register renaming and one move).
+The first add-with-carry of each column writes c2 from zero instead of
+adding into it, which saves a zeroing move per column (15 per product: on a
+lone wave the unrolled Skyscraper compression went 30.3k -> 28.5k cycles)."""
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -29,15 +32,17 @@ def column(k):
     terms += [(f"m[{j}]", f"p_limb({k - j})", "s") for j in range(8) if j < k and 1 <= k - j < 8]
     body, ops = [], []
     n = 3  # %0 c01, %1 c2, %2 cy
-    for x, y, cls in terms:
+    for t, (x, y, cls) in enumerate(terms):
         body.append(f"v_mad_u64_u32 %0, %2, %{n}, %{n + 1}, %0")
-        body.append("v_addc_co_u32_e64 %1, %2, 0, %1, %2")
+        # the column's first carry starts c2 (0 + 0 + carry): no zeroing move
+        body.append("v_addc_co_u32_e64 %1, %2, 0, 0, %2" if t == 0 else
+                    "v_addc_co_u32_e64 %1, %2, 0, %1, %2")
         ops += [f'"v"({x})', f'"{cls}"({y})']
         n += 2
     out = []
     if body:
         text = "\\n\\t".join(body)
-        out.append(f'  asm("{text}"\n               : "+v"(c01), "+v"(c2), "=&s"(cy)\n'
+        out.append(f'  asm("{text}"\n               : "+v"(c01), "=&v"(c2), "=&s"(cy)\n'
                    f'               : {", ".join(ops)});')
     if k < 8:
         # m_k from the column's low word, then m_k * p0 into the column
@@ -48,7 +53,6 @@ def column(k):
     else:
         out.append(f"  r.l[{k - 8}] = (uint32_t)c01;")
     out.append("  c01 = (c01 >> 32) | ((uint64_t)c2 << 32);")
-    out.append("  c2 = 0;")
     return out
 
 

@@ -47,18 +47,24 @@ __device__ __forceinline__ uint32_t p4_limb(int i) {
     case 6: return 0x84c680a6u; default: return 0xc19139cbu;
   }
 }
-__device__ __forceinline__ void cond_sub_4p(fr_t& a) {
-  fr_t d;
-  uint32_t br = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) d.l[i] = subb(a.l[i], p4_limb(i), br);
-#pragma unroll
-  for (int i = 0; i < 8; i++) a.l[i] = br ? a.l[i] : d.l[i];
-}
-// any 256-bit integer (< 2^256 < 6p) -> [0, 2p)
+// any 256-bit integer (< 2^256 < 5.3p) -> [0, 2p): a - 4p if that is not
+// negative (< 1.3p), else a - 2p if not negative, else a.  The two borrow
+// chains are independent, so a lone wave fills the wait states of one with
+// the other (952 -> 856 cycles per bar on one wave, same instruction count as
+// subtracting 4p and then 2p in turn).
 __device__ __forceinline__ void reduce_any(fr_t& a) {
-  cond_sub_4p(a);
-  fr_reduce2p(a);
+  fr_t d2, d4;
+  uint32_t b2 = 0, b4 = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    d2.l[i] = subb(a.l[i], p2_limb(i), b2);
+    d4.l[i] = subb(a.l[i], p4_limb(i), b4);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint32_t x = b2 ? a.l[i] : d2.l[i];
+    a.l[i] = b4 ? x : d4.l[i];
+  }
 }
 
 // Byte S-box on the four bytes of a word: rotl1(v ^ (rotl1(~v) & rotl2(v) & rotl3(v))).
@@ -88,26 +94,48 @@ __device__ __forceinline__ fr_t sky_rc(int round) {
   return c;
 }
 
-// compress(l, r) for arbitrary 256-bit inputs; canonical output.  Written as
-// the reference's nine double rounds (ss / bb, reference.rs:49-78): two
+// Double round dr of the reference's nine (ss / bb, reference.rs:49-78): two
 // Feistel rounds update r then l in place, so nothing is swapped.
-__device__ __forceinline__ fr_t sky_compress(fr_t l, fr_t r) {
+__device__ __forceinline__ void sky_double_round(fr_t& l, fr_t& r, int dr) {
+  const bool bb = (dr == 3) | (dr == 5);  // rounds 6, 7 and 10, 11
+  const fr_t f0 = bb ? sky_bar(l) : fr_sqr(l);
+  r = fr_add(r, f0);
+  if (dr != 0) r = fr_add(r, sky_rc(2 * dr));  // rc[0] = 0
+  const fr_t f1 = bb ? sky_bar(r) : fr_sqr(r);
+  l = fr_add(l, f1);
+  if (dr != 8) l = fr_add(l, sky_rc(2 * dr + 1));  // rc[17] = 0
+}
+
+// compress(l, r) for arbitrary 256-bit inputs; canonical output.
+// UNROLLED: the nine double rounds as straight-line code (~6.7k instructions,
+// ~40 KB): no branches, the round constants as literals, and the scheduler
+// works across rounds.  On one wave a compression takes 28.5k cycles against
+// 35.3k for the loop, and the full-occupancy rate is ~2 % higher; kernels
+// with one call site use it.  Straight-line code starting cold costs little
+// (~0.3 us per launch, measured), but several inlined copies in a kernel
+// whose waves all stream through them would overflow the 64 KB instruction
+// cache, so such kernels take the loop (UNROLLED = false).
+template <bool UNROLLED>
+__device__ __forceinline__ fr_t sky_compress_t(fr_t l, fr_t r) {
   reduce_any(l);
   reduce_any(r);
   const fr_t t = l;
+  if constexpr (UNROLLED) {
+#pragma unroll
+    for (int dr = 0; dr < 9; dr++) sky_double_round(l, r, dr);
+  } else {
 #pragma unroll 1
-  for (int dr = 0; dr < 9; dr++) {
-    const bool bb = (dr == 3) | (dr == 5);  // rounds 6, 7 and 10, 11
-    const fr_t f0 = bb ? sky_bar(l) : fr_sqr(l);
-    r = fr_add(r, f0);
-    if (dr != 0) r = fr_add(r, sky_rc(2 * dr));  // rc[0] = 0
-    const fr_t f1 = bb ? sky_bar(r) : fr_sqr(r);
-    l = fr_add(l, f1);
-    if (dr != 8) l = fr_add(l, sky_rc(2 * dr + 1));  // rc[17] = 0
+    for (int dr = 0; dr < 9; dr++) sky_double_round(l, r, dr);
   }
   fr_t o = fr_add(l, t);
   fr_canon(o);
   return o;
+}
+__device__ __forceinline__ fr_t sky_compress(const fr_t& l, const fr_t& r) {
+  return sky_compress_t<true>(l, r);
+}
+__device__ __forceinline__ fr_t sky_compress_rolled(const fr_t& l, const fr_t& r) {
+  return sky_compress_t<false>(l, r);
 }
 
 // SkyscraperHashEngine::hash_many of one message of `chunks` words
@@ -117,21 +145,37 @@ __device__ __forceinline__ fr_t sky_compress(fr_t l, fr_t r) {
 // per compression would touch every line four times far apart in time, and
 // the lines get evicted in between.  Four compressions take tens of
 // microseconds per wave, so the other waves of the SIMD hide the next load.
-__device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks) {
+// The compressions of a group run in a rolled loop over a shifting window,
+// so the kernel holds one copy of the compression.
+template <bool UNROLLED>
+__device__ __forceinline__ fr_t sky_hash_message_t(const fr_t* m, uint64_t chunks) {
   constexpr int G = 4;
-  fr_t acc;
+  fr_t acc = {};
   for (uint64_t g = 0; g < chunks; g += G) {
-    fr_t w[G];
+    fr_t w[G] = {};
 #pragma unroll
     for (int j = 0; j < G; j++)
       if (g + j < chunks) w[j] = fr_load(m + g + j);
-#pragma unroll
-    for (int j = 0; j < G; j++) {
-      if (g + j >= chunks) break;
-      acc = (g + j == 0) ? w[0] : sky_compress(acc, w[j]);
+    int n = chunks - g < (uint64_t)G ? (int)(chunks - g) : G;
+    if (g == 0) {  // m0 starts the fold
+      acc = w[0];
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
+      n -= 1;
+    }
+#pragma unroll 1
+    for (int j = 0; j < n; j++) {
+      acc = sky_compress_t<UNROLLED>(acc, w[0]);
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
     }
   }
   return acc;
+}
+__device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks) {
+  return sky_hash_message_t<true>(m, chunks);
 }
 
 // Wave-level helpers for task loops whose waves work independently (no
