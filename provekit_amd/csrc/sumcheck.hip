@@ -12,8 +12,8 @@
 //    LDS, one partial per workgroup.
 //  * k_sc_finalize: sums the partials, forms the round polynomial
 //    (whir_r1cs.rs:567-609, with compute_blinding_coefficients_for_round
-//    :427-492), runs the transcript step and writes the next challenge, all on
-//    one wave, so a whole sumcheck runs without the host.
+//    :427-492), runs the transcript step and writes the next challenge, in
+//    one workgroup, so a whole sumcheck runs without the host.
 // The transcript is the project's Skyscraper transcript (init / absorb1 /
 // round step / squeeze with parallel compressions, defined in sumcheck.hpp).
 #include "host_fr.hpp"
@@ -485,32 +485,51 @@ __global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_
   }
 }
 
+// What thread 0 of a finalizing workgroup reads from ScState: loaded once at
+// kernel start, next to the partial sums, instead of after them (a lone
+// wave would wait out each load), and carried in registers across the rounds
+// of k_sc_tail.
+struct RoundCtx {
+  fr_t state, S, half;
+  int mode;
+};
+__device__ __forceinline__ RoundCtx load_ctx(const ScState* st) {
+  RoundCtx c;
+  c.state = st->state;
+  c.S = st->S;
+  c.half = st->half;
+  c.mode = st->mode;
+  return c;
+}
+
 // The end of round idx, run by every thread of the calling workgroup (it
-// holds barriers): h0, hm1, hinf are the round's sums (read on thread 0).
-// With `claim_hi` (round 0 of the plain mode) thread 0 first takes
-// S = claim = h(0) + h(1) and absorbs it (the order of k_sc_claim_finalize).
-// Then: form the cubic, check c0 + c0 + c1 + c2 + c3 = S, the transcript
-// round step (the new state is alpha_idx), S = cubic(alpha).  Returns
-// alpha_idx on thread 0.
-__device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t& hm1,
-                               const fr_t& hinf, const fr_t* claim_hi) {
+// holds barriers): h0, hm1, hinf are the round's sums and ctx the round's
+// state (both read on thread 0).  With `claim_hi` (round 0 of the plain mode)
+// thread 0 first takes S = claim = h(0) + h(1) and absorbs it (the order of
+// k_sc_claim_finalize).  Then: form the cubic, check c0 + c0 + c1 + c2 + c3 =
+// S, the transcript round step (the new state is alpha_idx), S = cubic(alpha).
+// Thread 0 leaves the new state and S in ctx and in st.
+__device__ __forceinline__ void finalize_round(ScState* st, int idx, RoundCtx& ctx, const fr_t& h0,
+                                               const fr_t& hm1, const fr_t& hinf,
+                                               const fr_t* claim_hi) {
   __shared__ fr_t co[4];
   __shared__ fr_t hh[2];
   __shared__ fr_t state;
   const int t = threadIdx.x;
   if (t == 0) {
-    fr_t stt = st->state;
+    fr_t stt = ctx.state;
     if (claim_hi) {
       fr_t cl = fr_add(h0, *claim_hi);
       fr_canon(cl);
+      ctx.S = cl;
       st->S = cl;
       st->rho = fr_zero();
       stt = sky_compress(stt, cl);
     }
     state = stt;
-    const fr_t S = st->S;
+    const fr_t S = ctx.S;
     fr_t c0 = h0, at_m1 = hm1, c3 = hinf;
-    if (st->mode == 1) {
+    if (ctx.mode == 1) {
       // the blinding polynomial's part, rho * g (plain mode: rho = 0)
       fr_t g[4];
       blinding_coeffs(st, idx, g);
@@ -521,12 +540,16 @@ __device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t&
       c3 = fr_add(c3, fr_mul(rho, g[3]));
     }
     const fr_t three_c0 = fr_add(fr_add(c0, c0), c0);
-    fr_t c2 = fr_mul(st->half, fr_sub(fr_add(S, at_m1), three_c0));
+    fr_t c2 = fr_mul(ctx.half, fr_sub(fr_add(S, at_m1), three_c0));
     fr_t c1 = fr_sub(fr_sub(fr_sub(fr_sub(S, c0), c0), c3), c2);
     fr_canon(c0);
     fr_canon(c1);
     fr_canon(c2);
     fr_canon(c3);
+    co[0] = c0;
+    co[1] = c1;
+    co[2] = c2;
+    co[3] = c3;
     fr_t chk = fr_add(fr_add(fr_add(fr_add(c0, c0), c1), c2), c3);
     fr_canon(chk);
     fr_t Sc = S;
@@ -535,10 +558,6 @@ __device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t&
 #pragma unroll
     for (int q = 0; q < 8; q++) ok &= chk.l[q] == Sc.l[q];
     if (!ok) st->error = 1;
-    co[0] = c0;
-    co[1] = c1;
-    co[2] = c2;
-    co[3] = c3;
     st->coeffs[idx][0] = c0;
     st->coeffs[idx][1] = c1;
     st->coeffs[idx][2] = c2;
@@ -555,55 +574,76 @@ __device__ fr_t finalize_round(ScState* st, int idx, const fr_t& h0, const fr_t&
     hh[t] = sky_compress(x, y);
   }
   __syncthreads();
-  fr_t alpha = fr_zero();
   if (t == 0) {
-    alpha = sky_compress(hh[0], hh[1]);
+    const fr_t alpha = sky_compress(hh[0], hh[1]);
     st->alpha[idx] = alpha;
     st->state = alpha;
     const fr_t cf[4] = {co[0], co[1], co[2], co[3]};
-    st->S = cubic(cf, alpha);
+    const fr_t S = cubic(cf, alpha);
+    st->S = S;
+    ctx.state = alpha;
+    ctx.S = S;
   }
-  return alpha;
 }
 
-// Sum of np partials of NV values (stride NV) over the 64 lanes of one wave;
-// the result is read on lane 0.
+// Sum of np <= kMaxRoundBlocks partials of NV values (stride NV) by a
+// workgroup of kFinThreads: every thread issues its loads (at most four
+// partials) together, then wave sums by shuffles and the per-wave sums on
+// thread 0, where the result is read.  One wave looping over the partials
+// had waited out up to sixteen dependent loads (~30 us at 1024 partials).
+constexpr int kFinThreads = 256;
 template <int NV>
-__device__ __forceinline__ void wave_sum_partials(const fr_t* partials, unsigned np,
-                                                  fr_t (&out)[NV]) {
-  __shared__ fr_t red[NV][64];
-  const int t = threadIdx.x;
+__device__ __forceinline__ void block_sum_partials(const fr_t* partials, unsigned np,
+                                                   fr_t (&out)[NV]) {
+  constexpr int K = kMaxRoundBlocks / kFinThreads;
+  __shared__ fr_t wsum[NV][kFinThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  fr_t x[K][NV];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const unsigned i = t + k * kFinThreads;
+#pragma unroll
+    for (int q = 0; q < NV; q++) x[k][q] = i < np ? fr_load(partials + NV * i + q) : fr_zero();
+  }
   fr_t acc[NV];
 #pragma unroll
-  for (int q = 0; q < NV; q++) acc[q] = fr_zero();
-  for (unsigned i = t; i < np; i += 64) {
+  for (int q = 0; q < NV; q++) {
+    acc[q] = x[0][q];
 #pragma unroll
-    for (int q = 0; q < NV; q++) acc[q] = fr_add(acc[q], fr_load(partials + NV * i + q));
+    for (int k = 1; k < K; k++) acc[q] = fr_add(acc[q], x[k][q]);
   }
 #pragma unroll
-  for (int q = 0; q < NV; q++) red[q][t] = acc[q];
-  __syncthreads();
   for (int w = 32; w > 0; w >>= 1) {
-    if (t < w) {
 #pragma unroll
-      for (int q = 0; q < NV; q++) red[q][t] = fr_add(red[q][t], red[q][t + w]);
-    }
-    __syncthreads();
+    for (int q = 0; q < NV; q++) acc[q] = fr_add(acc[q], fr_shfl(acc[q], lane ^ w));
   }
+  if (lane == 0) {
 #pragma unroll
-  for (int q = 0; q < NV; q++) out[q] = red[q][0];
+    for (int q = 0; q < NV; q++) wsum[q][wave] = acc[q];
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      out[q] = wsum[q][0];
+#pragma unroll
+      for (int w = 1; w < kFinThreads / 64; w++) out[q] = fr_add(out[q], wsum[q][w]);
+    }
+  }
 }
 
 // Round idx after k_sc_round: sum the np workgroup partials, then
 // finalize_round.  NV = 4: round 0 of the plain mode, whose partials also
 // carry the high-half sums of the claim.
 template <int NV>
-__global__ __launch_bounds__(64) void k_sc_finalize(ScState* st, const fr_t* partials, unsigned np,
-                                                    int idx) {
+__global__ __launch_bounds__(kFinThreads) void k_sc_finalize(ScState* st, const fr_t* partials,
+                                                             unsigned np, int idx) {
+  RoundCtx ctx;
+  if (threadIdx.x == 0) ctx = load_ctx(st);
   fr_t sums[NV];
-  wave_sum_partials<NV>(partials, np, sums);
+  block_sum_partials<NV>(partials, np, sums);
   const fr_t hi = sums[NV - 1];
-  finalize_round(st, idx, sums[0], sums[1], sums[2], NV == 4 ? &hi : nullptr);
+  finalize_round(st, idx, ctx, sums[0], sums[1], sums[2], NV == 4 ? &hi : nullptr);
 }
 
 // The last rounds, from an input of cur <= kTailElems elements down to 2, in
@@ -621,7 +661,13 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
   __shared__ fr_t wsum[3][kTailThreads / 64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   fr_t* mle[4] = {a, b, c, eq};
-  if (t == 0) alpha_sh = st->alpha[idx0 - 1];
+  // in LDS, not registers: the tail's fold and the unrolled compressions
+  // already fill the 256 VGPRs a 512-thread workgroup allows
+  __shared__ RoundCtx ctx;
+  if (t == 0) {
+    ctx = load_ctx(st);
+    alpha_sh = ctx.state;  // the state after round idx0 - 1 is its challenge
+  }
   __syncthreads();
   const int m = st->m;
   for (int idx = idx0; idx < m; idx++, cur >>= 1) {
@@ -673,8 +719,8 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
 #pragma unroll
       for (int q = 0; q < 3; q++) fr_canon(h3[q]);
     }
-    const fr_t al = finalize_round(st, idx, h3[0], h3[1], h3[2], nullptr);
-    if (t == 0) alpha_sh = al;
+    finalize_round(st, idx, ctx, h3[0], h3[1], h3[2], nullptr);
+    if (t == 0) alpha_sh = ctx.state;
     __syncthreads();
   }
 }
@@ -778,14 +824,14 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
       break;
     }
     // two launches per round: fold + map + workgroup sums, then the
-    // one-wave sum of the partials and transcript step
+    // sum of the partials and the transcript step in one workgroup
     const unsigned nb = round_blocks(fold ? cur / 4 : cur / 2);
     if (idx == 0 && mode == 0) {
       k_sc_round<true><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
-      k_sc_finalize<4><<<1, 64, 0, s>>>(st, partials, nb, idx);
+      k_sc_finalize<4><<<1, kFinThreads, 0, s>>>(st, partials, nb, idx);
     } else {
       k_sc_round<false><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
-      k_sc_finalize<3><<<1, 64, 0, s>>>(st, partials, nb, idx);
+      k_sc_finalize<3><<<1, kFinThreads, 0, s>>>(st, partials, nb, idx);
     }
     if (fold) cur /= 2;
     PK_HIP(hipGetLastError());
