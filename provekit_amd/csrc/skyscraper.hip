@@ -22,8 +22,11 @@ __global__ __launch_bounds__(256) void k_compress_many(const fr_t* pairs, fr_t* 
   fr_store(out + i, sky_compress(l, r));
 }
 
-// One lane per message of `chunks` words.
-__global__ __launch_bounds__(256) void k_hash_fold(const fr_t* in, uint64_t chunks, uint64_t count,
+// One lane per message of `chunks` words.  At most 128 VGPRs (four waves
+// per SIMD, the whole 2^18-leaf grid resident at once): left to itself the
+// compiler spends 167 on the unrolled compression and the word window, which
+// fits three.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hash_fold(const fr_t* in, uint64_t chunks, uint64_t count,
                                                    fr_t* out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
