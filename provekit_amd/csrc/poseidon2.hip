@@ -68,21 +68,37 @@ __device__ __forceinline__ void ext_mds29(f29 s[4]) {
   for (int i = 0; i < 4; i++) norm29(s[i]);
 }
 
-__device__ __noinline__ void p2_permute29(fr_t st[4]) {
+// One full round on the four lanes, written out lane by lane: as a loop over
+// the lanes inside the rolled round loop the compiler left the lane index
+// dynamic and kept the state in scratch memory (private segment 304 bytes,
+// 116 VGPRs; written out: none, 76).
+__device__ __forceinline__ f29 rc_sbox29(const f29& v, const f29& rc) {
+  f29 x = add29(v, rc);
+  norm29(x);
+  return sbox5_29(x);
+}
+__device__ __forceinline__ void full_round29(f29 s[4], int r) {
+  const f29 s0 = rc_sbox29(s[0], kP2RC29[r][0]);
+  const f29 s1 = rc_sbox29(s[1], kP2RC29[r][1]);
+  const f29 s2 = rc_sbox29(s[2], kP2RC29[r][2]);
+  const f29 s3 = rc_sbox29(s[3], kP2RC29[r][3]);
+  s[0] = s0;
+  s[1] = s1;
+  s[2] = s2;
+  s[3] = s3;
+  ext_mds29(s);
+}
+
+// Inlined into each kernel, which calls it from one place (~7.3k
+// instructions, 44 KB): an out-of-line call passes the state through scratch
+// memory.
+__device__ __forceinline__ void p2_permute29(fr_t st[4]) {
   f29 s[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) s[i] = mul29(to29(st[i]), kP2In29);
   ext_mds29(s);
 #pragma unroll 1
-  for (int r = 0; r < 4; r++) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      f29 x = add29(s[i], kP2RC29[r][i]);
-      norm29(x);
-      s[i] = sbox5_29(x);
-    }
-    ext_mds29(s);
-  }
+  for (int r = 0; r < 4; r++) full_round29(s, r);
 #pragma unroll 1
   for (int r = 4; r < 60; r++) {
     f29 x = add29(s[0], kP2RC29[r][0]);
@@ -95,15 +111,7 @@ __device__ __noinline__ void p2_permute29(fr_t st[4]) {
     for (int i = 0; i < 4; i++) s[i] = add29(mul29(s[i], kP2Diag29[i]), sum);
   }
 #pragma unroll 1
-  for (int r = 60; r < 64; r++) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      f29 x = add29(s[i], kP2RC29[r][i]);
-      norm29(x);
-      s[i] = sbox5_29(x);
-    }
-    ext_mds29(s);
-  }
+  for (int r = 60; r < 64; r++) full_round29(s, r);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     f29 v = mul29(s[i], kP2Out29);  // < 1.2 p: one subtraction makes it canonical
@@ -148,14 +156,16 @@ __global__ __launch_bounds__(256) void k_p2_hash_many(const fr_t* in, uint64_t l
   s[3] = iv;
   uint64_t pos = 0;
   // every full block of 3 is added then permuted; the tail (1..3 elements,
-  // or none for an empty message) is added before the final permutation
-  while (len - pos > 3) {
-    for (int k = 0; k < 3; k++) s[k] = fr_add(s[k], fr_load(m + pos + k));
+  // or none for an empty message) is added before the final permutation.
+  // One loop, so the kernel holds one copy of the permutation.
+  do {
+    const uint64_t take = len - pos > 3 ? 3 : len - pos;
+#pragma unroll
+    for (int k = 0; k < 3; k++)  // static indices keep the state in registers
+      if ((uint64_t)k < take) s[k] = fr_add(s[k], fr_load(m + pos + k));
+    pos += take;
     p2_permute(s);
-    pos += 3;
-  }
-  for (uint64_t k = 0; pos + k < len; k++) s[k] = fr_add(s[k], fr_load(m + pos + k));
-  p2_permute(s);
+  } while (pos < len);
   fr_store(out + i, s[0]);
 }
 
