@@ -224,8 +224,29 @@ __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   return r;
 }
 
-// a lazy -> a^2*2^-256 lazy.
-__device__ __forceinline__ fr_t fr_sqr(const fr_t& a) { return fr_mul(a, a); }
+// a lazy (any a < 2^255) -> a^2*2^-256 lazy, the value and bytes fr_mul(a, a)
+// gives.  Each cross product a_i a_j (i < j) is formed once, against the
+// doubled limbs of a: d[j] = (a_j << 1) | (a_{j-1} >> 31) and, next to the
+// diagonal, u[j] = a_j << 1 (fr_sqr_asm.inc, written by gen_fr_mul.py, which
+// states the identity): 36 digit products for the a*a half instead of 64,
+// ~40 fewer instructions per squaring, 14 squarings per Skyscraper
+// compression.
+__device__ __forceinline__ fr_t fr_sqr(const fr_t& a) {
+  uint32_t u[8], d[8];
+#pragma unroll
+  for (int j = 1; j < 8; j++) {
+    u[j] = a.l[j] << 1;
+    d[j] = __builtin_amdgcn_alignbit(a.l[j], a.l[j - 1], 31);
+  }
+  uint32_t m[8];
+  fr_t r;
+  uint64_t c01 = 0;
+  uint32_t c2 = 0;
+  uint64_t cy;
+#include "fr_sqr_asm.inc"
+  (void)cy;
+  return r;
+}
 
 __device__ __forceinline__ bool fr_is_zero(const fr_t& a) {
   uint32_t o = 0;

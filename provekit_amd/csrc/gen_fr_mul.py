@@ -1,6 +1,8 @@
-"""Writes fr_mul_asm.inc, the body of fr_mul in fr.hpp: the 8 x 32-bit
-Montgomery product with one inline-asm statement per column (run:
-python3 gen_fr_mul.py, from this folder).
+"""Writes fr_mul_asm.inc and fr_sqr_asm.inc, the bodies of fr_mul and fr_sqr
+in fr.hpp: the 8 x 32-bit Montgomery product with one inline-asm statement
+per column (run: python3 gen_fr_mul.py, from this folder).  The squaring
+forms each cross product of a * a once against the doubled limbs of a (36
+digit products instead of 64; see square_terms).
 
 Why per column: the compiler cannot see into inline asm, so after every asm
 statement it inserts a wait state before the next use of the statement's
@@ -27,8 +29,28 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def column(k):
-    terms = [(f"a.l[{i}]", f"b.l[{k - i}]", "v") for i in range(8) if 0 <= k - i < 8]
+def square_terms(k):
+    """Column k of a*a with each cross product once: row i takes the doubled
+    limbs of a above it, d[j] = (a_j << 1) | (a_{j-1} >> 31) for j >= i + 2
+    and u[j] = a_j << 1 for j = i + 1 (a_i's own top bit belongs to row i's
+    diagonal side, not to 2 * A_{>i}).  Then sum_i a_i * 2 * A_{>i} equals the
+    cross-product sum exactly, so every column, quotient digit and output is
+    the one fr_mul(a, a) forms.  Needs a < 2^255 (2a fits in eight limbs)."""
+    terms = []
+    for i in range(8):
+        j = k - i
+        if i < j <= 7:
+            terms.append((f"a.l[{i}]", f"u[{j}]" if j == i + 1 else f"d[{j}]", "v"))
+    if k % 2 == 0 and k // 2 <= 7:
+        terms.append((f"a.l[{k // 2}]", f"a.l[{k // 2}]", "v"))
+    return terms
+
+
+def column(k, square=False):
+    if square:
+        terms = square_terms(k)
+    else:
+        terms = [(f"a.l[{i}]", f"b.l[{k - i}]", "v") for i in range(8) if 0 <= k - i < 8]
     terms += [(f"m[{j}]", f"p_limb({k - j})", "s") for j in range(8) if j < k and 1 <= k - j < 8]
     body, ops = [], []
     n = 3  # %0 c01, %1 c2, %2 cy
@@ -56,15 +78,23 @@ def column(k):
     return out
 
 
-def main():
-    lines = ["// Generated by gen_fr_mul.py; do not edit.  The body of fr_mul (fr.hpp):",
-             "// in a, b (lazy); out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there."]
+def write(name, head, square):
+    lines = [f"// Generated by gen_fr_mul.py; do not edit.  {head}"]
     for k in range(15):
         lines.append(f"  // column {k}")
-        lines += column(k)
+        lines += column(k, square)
     lines.append("  r.l[7] = (uint32_t)c01;")
-    with open(os.path.join(HERE, "fr_mul_asm.inc"), "w") as f:
+    with open(os.path.join(HERE, name), "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def main():
+    write("fr_mul_asm.inc", "The body of fr_mul (fr.hpp):\n"
+          "// in a, b (lazy); out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there.",
+          False)
+    write("fr_sqr_asm.inc", "The body of fr_sqr (fr.hpp):\n"
+          "// in a (< 2^255) and its doubled limbs u, d; out r = a * a * 2^-256 (lazy);\n"
+          "// m[8], c01, c2, cy declared there.", True)
 
 
 if __name__ == "__main__":
