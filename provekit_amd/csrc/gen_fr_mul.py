@@ -46,12 +46,19 @@ def square_terms(k):
     return terms
 
 
-def column(k, square=False):
+def column_terms(k, square=False):
+    """The digit products summed in column k before its quotient digit (for
+    k < 8 the asm then adds m_k * p_0): a*b (or a*a) and m_j * p_{k-j}."""
     if square:
         terms = square_terms(k)
     else:
         terms = [(f"a.l[{i}]", f"b.l[{k - i}]", "v") for i in range(8) if 0 <= k - i < 8]
     terms += [(f"m[{j}]", f"p_limb({k - j})", "s") for j in range(8) if j < k and 1 <= k - j < 8]
+    return terms
+
+
+def column(k, square=False):
+    terms = column_terms(k, square)
     body, ops = [], []
     n = 3  # %0 c01, %1 c2, %2 cy
     for t, (x, y, cls) in enumerate(terms):
