@@ -107,14 +107,15 @@ __device__ __forceinline__ void sky_double_round(fr_t& l, fr_t& r, int dr) {
 }
 
 // compress(l, r) for arbitrary 256-bit inputs; canonical output.
-// UNROLLED: the nine double rounds as straight-line code (~6.7k instructions,
-// ~40 KB): no branches, the round constants as literals, and the scheduler
-// works across rounds.  On one wave a compression takes 25.7k cycles against
-// 35.3k for the loop (with the interleaved additions of fr_add_ilv), and the full-occupancy rate is ~2 % higher; kernels
-// with one call site use it.  Straight-line code starting cold costs little
-// (~0.3 us per launch, measured), but several inlined copies in a kernel
-// whose waves all stream through them would overflow the 64 KB instruction
-// cache, so such kernels take the loop (UNROLLED = false).
+// UNROLLED: the nine double rounds as straight-line code (~5.6k instructions,
+// ~34 KB): no branches, the round constants as literals, and the scheduler
+// works across rounds.  On one wave the unrolled form took 30.3k cycles
+// against 35.3k for the loop (22.7k now, with fr_add_ilv and fr_sqr), and
+// its full-occupancy rate is ~2 % higher; kernels with one call site use it.
+// Straight-line code starting cold costs little (~0.3 us per launch,
+// measured), but several inlined copies in a kernel whose waves all stream
+// through them would overflow the 64 KB instruction cache, so such kernels
+// take the loop (UNROLLED = false).
 template <bool UNROLLED>
 __device__ __forceinline__ fr_t sky_compress_t(fr_t l, fr_t r) {
   reduce_any(l);
