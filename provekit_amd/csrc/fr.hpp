@@ -140,6 +140,51 @@ __device__ __forceinline__ fr_t fr_add_ilv(const fr_t& a, const fr_t& b) {
   (void)c;
   return r;
 }
+// lazy - lazy -> lazy, the fr_add_ilv scheme for a subtraction: the
+// difference's borrow chain (SGPR pair) interleaved with the chain that adds
+// 2p back to it (vcc), then the final borrow picks the one that is in range.
+// 24 instructions like fr_sub; for latency-bound code.
+__device__ __forceinline__ fr_t fr_sub_ilv(const fr_t& a, const fr_t& b) {
+  fr_t r;
+  uint32_t u[8];
+  uint64_t c;
+  asm(
+      "v_sub_co_u32_e64 %0, %16, %17, %25\n\t"
+      "v_subb_co_u32_e64 %1, %16, %18, %26, %16\n\t"
+      "v_add_co_u32_e32 %8, vcc, 0xe0000002, %0\n\t"
+      "v_subb_co_u32_e64 %2, %16, %19, %27, %16\n\t"
+      "v_addc_co_u32_e32 %9, vcc, %33, %1, vcc\n\t"
+      "v_subb_co_u32_e64 %3, %16, %20, %28, %16\n\t"
+      "v_addc_co_u32_e32 %10, vcc, %34, %2, vcc\n\t"
+      "v_subb_co_u32_e64 %4, %16, %21, %29, %16\n\t"
+      "v_addc_co_u32_e32 %11, vcc, %35, %3, vcc\n\t"
+      "v_subb_co_u32_e64 %5, %16, %22, %30, %16\n\t"
+      "v_addc_co_u32_e32 %12, vcc, %36, %4, vcc\n\t"
+      "v_subb_co_u32_e64 %6, %16, %23, %31, %16\n\t"
+      "v_addc_co_u32_e32 %13, vcc, %37, %5, vcc\n\t"
+      "v_subb_co_u32_e64 %7, %16, %24, %32, %16\n\t"
+      "v_addc_co_u32_e32 %14, vcc, %38, %6, vcc\n\t"
+      "v_addc_co_u32_e32 %15, vcc, %39, %7, vcc\n\t"
+      "v_cndmask_b32_e64 %0, %0, %8, %16\n\t"
+      "v_cndmask_b32_e64 %1, %1, %9, %16\n\t"
+      "v_cndmask_b32_e64 %2, %2, %10, %16\n\t"
+      "v_cndmask_b32_e64 %3, %3, %11, %16\n\t"
+      "v_cndmask_b32_e64 %4, %4, %12, %16\n\t"
+      "v_cndmask_b32_e64 %5, %5, %13, %16\n\t"
+      "v_cndmask_b32_e64 %6, %6, %14, %16\n\t"
+      "v_cndmask_b32_e64 %7, %7, %15, %16"
+      : "=&v"(r.l[0]), "=&v"(r.l[1]), "=&v"(r.l[2]), "=&v"(r.l[3]), "=&v"(r.l[4]), "=&v"(r.l[5]), "=&v"(r.l[6]), "=&v"(r.l[7]),
+        "=&v"(u[0]), "=&v"(u[1]), "=&v"(u[2]), "=&v"(u[3]), "=&v"(u[4]), "=&v"(u[5]), "=&v"(u[6]), "=&v"(u[7]), "=&s"(c)
+      : "v"(a.l[0]), "v"(a.l[1]), "v"(a.l[2]), "v"(a.l[3]),
+        "v"(a.l[4]), "v"(a.l[5]), "v"(a.l[6]), "v"(a.l[7]),
+        "v"(b.l[0]), "v"(b.l[1]), "v"(b.l[2]), "v"(b.l[3]),
+        "v"(b.l[4]), "v"(b.l[5]), "v"(b.l[6]), "v"(b.l[7]),
+        "v"(p2_limb(1)), "v"(p2_limb(2)), "v"(p2_limb(3)), "v"(p2_limb(4)), "v"(p2_limb(5)), "v"(p2_limb(6)), "v"(p2_limb(7))
+      : "vcc");
+  (void)c;
+  return r;
+}
+
 // lazy - lazy -> lazy  (a - b, adding 2p back on borrow)
 __device__ __forceinline__ fr_t fr_sub(const fr_t& a, const fr_t& b) {
   fr_t r;

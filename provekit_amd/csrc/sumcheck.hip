@@ -386,9 +386,9 @@ __global__ __launch_bounds__(kRoundThreads) void k_sum_partials(const fr_t* part
 // ------------------------------------------------------------------ driver kernels
 __device__ __forceinline__ fr_t cubic(const fr_t* p, const fr_t& x) {
   fr_t v = p[3];
-  v = fr_add(p[2], fr_mul(x, v));
-  v = fr_add(p[1], fr_mul(x, v));
-  v = fr_add(p[0], fr_mul(x, v));
+  v = fr_add_ilv(p[2], fr_mul(x, v));
+  v = fr_add_ilv(p[1], fr_mul(x, v));
+  v = fr_add_ilv(p[0], fr_mul(x, v));
   fr_canon(v);
   return v;
 }
@@ -519,7 +519,7 @@ __device__ __forceinline__ void finalize_round(ScState* st, int idx, RoundCtx& c
   if (t == 0) {
     fr_t stt = ctx.state;
     if (claim_hi) {
-      fr_t cl = fr_add(h0, *claim_hi);
+      fr_t cl = fr_add_ilv(h0, *claim_hi);
       fr_canon(cl);
       ctx.S = cl;
       st->S = cl;
@@ -534,14 +534,14 @@ __device__ __forceinline__ void finalize_round(ScState* st, int idx, RoundCtx& c
       fr_t g[4];
       blinding_coeffs(st, idx, g);
       const fr_t rho = st->rho;
-      const fr_t gm1 = fr_sub(fr_add(fr_sub(g[0], g[1]), g[2]), g[3]);
-      c0 = fr_add(c0, fr_mul(rho, g[0]));
-      at_m1 = fr_add(at_m1, fr_mul(rho, gm1));
-      c3 = fr_add(c3, fr_mul(rho, g[3]));
+      const fr_t gm1 = fr_sub_ilv(fr_add_ilv(fr_sub_ilv(g[0], g[1]), g[2]), g[3]);
+      c0 = fr_add_ilv(c0, fr_mul(rho, g[0]));
+      at_m1 = fr_add_ilv(at_m1, fr_mul(rho, gm1));
+      c3 = fr_add_ilv(c3, fr_mul(rho, g[3]));
     }
-    const fr_t three_c0 = fr_add(fr_add(c0, c0), c0);
-    fr_t c2 = fr_mul(ctx.half, fr_sub(fr_add(S, at_m1), three_c0));
-    fr_t c1 = fr_sub(fr_sub(fr_sub(fr_sub(S, c0), c0), c3), c2);
+    const fr_t three_c0 = fr_add_ilv(fr_add_ilv(c0, c0), c0);
+    fr_t c2 = fr_mul(ctx.half, fr_sub_ilv(fr_add_ilv(S, at_m1), three_c0));
+    fr_t c1 = fr_sub_ilv(fr_sub_ilv(fr_sub_ilv(fr_sub_ilv(S, c0), c0), c3), c2);
     fr_canon(c0);
     fr_canon(c1);
     fr_canon(c2);
@@ -550,7 +550,7 @@ __device__ __forceinline__ void finalize_round(ScState* st, int idx, RoundCtx& c
     co[1] = c1;
     co[2] = c2;
     co[3] = c3;
-    fr_t chk = fr_add(fr_add(fr_add(fr_add(c0, c0), c1), c2), c3);
+    fr_t chk = fr_add_ilv(fr_add_ilv(fr_add_ilv(fr_add_ilv(c0, c0), c1), c2), c3);
     fr_canon(chk);
     fr_t Sc = S;
     fr_canon(Sc);
@@ -610,12 +610,12 @@ __device__ __forceinline__ void block_sum_partials(const fr_t* partials, unsigne
   for (int q = 0; q < NV; q++) {
     acc[q] = x[0][q];
 #pragma unroll
-    for (int k = 1; k < K; k++) acc[q] = fr_add(acc[q], x[k][q]);
+    for (int k = 1; k < K; k++) acc[q] = fr_add_ilv(acc[q], x[k][q]);
   }
 #pragma unroll
   for (int w = 32; w > 0; w >>= 1) {
 #pragma unroll
-    for (int q = 0; q < NV; q++) acc[q] = fr_add(acc[q], fr_shfl(acc[q], lane ^ w));
+    for (int q = 0; q < NV; q++) acc[q] = fr_add_ilv(acc[q], fr_shfl(acc[q], lane ^ w));
   }
   if (lane == 0) {
 #pragma unroll
@@ -627,7 +627,7 @@ __device__ __forceinline__ void block_sum_partials(const fr_t* partials, unsigne
     for (int q = 0; q < NV; q++) {
       out[q] = wsum[q][0];
 #pragma unroll
-      for (int w = 1; w < kFinThreads / 64; w++) out[q] = fr_add(out[q], wsum[q][w]);
+      for (int w = 1; w < kFinThreads / 64; w++) out[q] = fr_add_ilv(out[q], wsum[q][w]);
     }
   }
 }
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
 #pragma unroll
     for (int w = 32; w > 0; w >>= 1) {
 #pragma unroll
-      for (int q = 0; q < 3; q++) v[q] = fr_add(v[q], fr_shfl(v[q], lane ^ w));
+      for (int q = 0; q < 3; q++) v[q] = fr_add_ilv(v[q], fr_shfl(v[q], lane ^ w));
     }
     if (lane == 0) {
 #pragma unroll
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
     if (t == 0) {
       for (int w = 0; w < kTailThreads / 64; w++) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) h3[q] = fr_add(h3[q], wsum[q][w]);
+        for (int q = 0; q < 3; q++) h3[q] = fr_add_ilv(h3[q], wsum[q][w]);
       }
 #pragma unroll
       for (int q = 0; q < 3; q++) fr_canon(h3[q]);
