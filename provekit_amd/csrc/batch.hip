@@ -59,15 +59,15 @@ __device__ __forceinline__ uint64_t level_off(uint64_t n0, uint32_t l) {
 // the dispatcher places one wave per SIMD per workgroup (one-wave
 // workgroups can stack several waves on one SIMD while another idles).  No
 // workgroup barrier: broadcasts and the tail's level exchange are wave-level.
-// The kernel has three compression call sites (leaf fold, node, tail) and
-// every wave streams through them, so it takes the rolled compression: three
-// unrolled copies would not fit the instruction cache.
+// Every task kind runs as a sequence of steps with one compression each, so
+// the kernel holds a single copy of the unrolled compression (three copies
+// would overflow the instruction cache); at most 128 VGPRs keep the 16 waves
+// per CU of the launch resident.
 constexpr int kQueueBlock = 256;
 
-__global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* polys,
-                                                            const HashTask* tasks, uint32_t ntasks,
-                                                            uint32_t* done, uint32_t* next,
-                                                            uint32_t* err) {
+__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_hash_queue(
+    const BatchPolyDev* polys, const HashTask* tasks, uint32_t ntasks, uint32_t* done,
+    uint32_t* next, uint32_t* err) {
   const int t = threadIdx.x & (kQueueWave - 1);
   for (;;) {
     const uint32_t id = wave_take(next);
@@ -81,38 +81,95 @@ __global__ __launch_bounds__(kQueueBlock) void k_hash_queue(const BatchPolyDev* 
     // wave.  Leaf tasks read only codewords written by earlier launches and
     // skip it.
     if (tk.dep0 >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (tk.kind == kTaskLeaf) {
-      const uint64_t leaf = (uint64_t)tk.j * kQueueWave + t;
-      if (leaf < p.leaves) {
-        fr_store(p.nodes + leaf,
-                          sky_hash_message_t<false>(p.codeword + leaf * p.leaf_elems, p.leaf_elems));
-      }
-    } else if (tk.kind == kTaskNode) {
+    // The task as steps of one compression each (wave-uniform count):
+    //  leaf: acc = m0, then acc = C(acc, m_k), k = 1 .. leaf_elems - 1, the
+    //        words read four at a time (one 128-byte line per lane; see
+    //        sky_hash_message) through a shifting window;
+    //  node: one step, C(in[2i], in[2i + 1]);
+    //  tail: level tk.level (<= 64 nodes, one per lane) up to the root, one
+    //        step per level, operands from the level below by shuffles.
+    const uint32_t kind = tk.kind;
+    uint64_t lane_i = (uint64_t)tk.j * kQueueWave + t;  // leaf or node of this lane
+    bool act = false;
+    int nsteps = 0;
+    fr_t acc = {}, w[4] = {};
+    int have = 0;            // words left in the window (leaf)
+    uint64_t next_word = 0;  // first word of the next window (leaf)
+    const fr_t* src = nullptr;
+    fr_t* dst = nullptr;
+    uint64_t n = 0;  // tail: nodes at the current level
+    if (kind == kTaskLeaf) {
+      act = lane_i < p.leaves;
+      src = p.codeword + (act ? lane_i : 0) * p.leaf_elems;
+      const uint64_t E = p.leaf_elems;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if ((uint64_t)q < E) w[q] = fr_load(src + q);
+      acc = w[0];
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
+      have = E < 4 ? (int)E - 1 : 3;
+      next_word = 4;
+      nsteps = (int)E - 1;
+      dst = p.nodes + lane_i;
+    } else if (kind == kTaskNode) {
       const uint64_t nl = p.leaves >> tk.level;
-      const uint64_t node = (uint64_t)tk.j * kQueueWave + t;
-      if (node < nl) {
-        const fr_t* in = p.nodes + level_off(p.leaves, tk.level - 1);
-        fr_store(p.nodes + level_off(p.leaves, tk.level) + node,
-                          sky_compress_rolled(fr_load(in + 2 * node),
-                                              fr_load(in + 2 * node + 1)));
-      }
-    } else {  // tail: level tk.level (<= 64 nodes, one per lane) up to the root
-      uint64_t n = p.leaves >> tk.level;
-      uint64_t off = level_off(p.leaves, tk.level);
-      fr_t cur = {};
-      if ((uint64_t)t < n) cur = fr_load(p.nodes + off + t);
-      off += n;
-      for (; n > 1; n >>= 1) {
-        const uint64_t h = n >> 1;
-        const fr_t a = fr_shfl(cur, (2 * t) & (kQueueWave - 1));
-        const fr_t b = fr_shfl(cur, (2 * t + 1) & (kQueueWave - 1));
-        if ((uint64_t)t < h) {
-          cur = sky_compress_rolled(a, b);
-          fr_store(p.nodes + off + t, cur);
+      act = lane_i < nl;
+      src = p.nodes + level_off(p.leaves, tk.level - 1) + 2 * (act ? lane_i : 0);
+      dst = p.nodes + level_off(p.leaves, tk.level) + lane_i;
+      nsteps = 1;
+    } else {
+      n = p.leaves >> tk.level;
+      const uint64_t off = level_off(p.leaves, tk.level);
+      if ((uint64_t)t < n) acc = fr_load(p.nodes + off + t);
+      dst = p.nodes + off + n;  // the next level starts here
+      nsteps = 0;
+      for (uint64_t m = n; m > 1; m >>= 1) nsteps++;
+    }
+#pragma unroll 1
+    for (int step = 0; step < nsteps; step++) {
+      fr_t x, y;
+      bool on;
+      if (kind == kTaskLeaf) {
+        if (have == 0) {  // the next four words (wave-uniform)
+          const uint64_t E = p.leaf_elems;
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (next_word + q < E) w[q] = fr_load(src + next_word + q);
+          have = E - next_word < 4 ? (int)(E - next_word) : 4;
+          next_word += 4;
         }
-        off += h;
+        x = acc;
+        y = w[0];
+        w[0] = w[1];
+        w[1] = w[2];
+        w[2] = w[3];
+        have--;
+        on = act;
+      } else if (kind == kTaskNode) {
+        x = fr_load(src);
+        y = fr_load(src + 1);
+        on = act;
+      } else {
+        x = fr_shfl(acc, (2 * t) & (kQueueWave - 1));
+        y = fr_shfl(acc, (2 * t + 1) & (kQueueWave - 1));
+        on = (uint64_t)t < (n >> 1);
+      }
+      fr_t v = {};
+      if (on) v = sky_compress(x, y);
+      if (kind == kTaskLeaf) {
+        acc = v;
+      } else if (kind == kTaskNode) {
+        if (on) fr_store(dst, v);
+      } else {
+        if (on) fr_store(dst + t, v);
+        acc = v;
+        dst += n >> 1;
+        n >>= 1;
       }
     }
+    if (kind == kTaskLeaf && act) fr_store(dst, acc);
     // publish: the wave's stores reach device-coherent memory (a release
     // fence writes back this XCD's L2; no invalidate) before the flag, which
     // every lane writes (one store of one address)
