@@ -106,37 +106,29 @@ __device__ __forceinline__ void sky_double_round(fr_t& l, fr_t& r, int dr) {
   if (dr != 8) l = fr_add_ilv(l, sky_rc(2 * dr + 1));  // rc[17] = 0
 }
 
-// compress(l, r) for arbitrary 256-bit inputs; canonical output.
-// UNROLLED: the nine double rounds as straight-line code (~5.6k instructions,
-// ~34 KB): no branches, the round constants as literals, and the scheduler
-// works across rounds.  On one wave the unrolled form took 30.3k cycles
-// against 35.3k for the loop (22.7k now, with fr_add_ilv and fr_sqr), and
-// its full-occupancy rate is ~2 % higher; kernels with one call site use it.
-// Straight-line code starting cold costs little (~0.3 us per launch,
-// measured), but several inlined copies in a kernel whose waves all stream
-// through them would overflow the 64 KB instruction cache, so such kernels
-// take the loop (UNROLLED = false).
-template <bool UNROLLED>
-__device__ __forceinline__ fr_t sky_compress_t(fr_t l, fr_t r) {
+// compress(l, r) for arbitrary 256-bit inputs; canonical output.  The nine
+// double rounds are straight-line code (~5.6k instructions, ~34 KB): no
+// branches, the round constants as literals, and the scheduler works across
+// rounds.  On one wave this took 30.3k cycles against 35.3k for a loop over
+// the double rounds (22.7k now, with fr_add_ilv and fr_sqr), and the
+// full-occupancy rate is ~2 % higher.  Straight-line code starting cold costs
+// little (~0.3 us per launch, measured), but several inlined copies in a
+// kernel whose waves all stream through them would overflow the 64 KB
+// instruction cache: every kernel here calls it from one place.  Two entry
+// forms of the same function: by value (the leaf fold's accumulator) and by
+// reference; each kernel keeps the form its measured code was built with.
+__device__ __forceinline__ fr_t sky_compress_v(fr_t l, fr_t r) {
   reduce_any(l);
   reduce_any(r);
   const fr_t t = l;
-  if constexpr (UNROLLED) {
 #pragma unroll
-    for (int dr = 0; dr < 9; dr++) sky_double_round(l, r, dr);
-  } else {
-#pragma unroll 1
-    for (int dr = 0; dr < 9; dr++) sky_double_round(l, r, dr);
-  }
+  for (int dr = 0; dr < 9; dr++) sky_double_round(l, r, dr);
   fr_t o = fr_add_ilv(l, t);
   fr_canon(o);
   return o;
 }
 __device__ __forceinline__ fr_t sky_compress(const fr_t& l, const fr_t& r) {
-  return sky_compress_t<true>(l, r);
-}
-__device__ __forceinline__ fr_t sky_compress_rolled(const fr_t& l, const fr_t& r) {
-  return sky_compress_t<false>(l, r);
+  return sky_compress_v(l, r);
 }
 
 // SkyscraperHashEngine::hash_many of one message of `chunks` words
@@ -148,8 +140,7 @@ __device__ __forceinline__ fr_t sky_compress_rolled(const fr_t& l, const fr_t& r
 // microseconds per wave, so the other waves of the SIMD hide the next load.
 // The compressions of a group run in a rolled loop over a shifting window,
 // so the kernel holds one copy of the compression.
-template <bool UNROLLED>
-__device__ __forceinline__ fr_t sky_hash_message_t(const fr_t* m, uint64_t chunks) {
+__device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks) {
   constexpr int G = 4;
   fr_t acc = {};
   for (uint64_t g = 0; g < chunks; g += G) {
@@ -167,16 +158,13 @@ __device__ __forceinline__ fr_t sky_hash_message_t(const fr_t* m, uint64_t chunk
     }
 #pragma unroll 1
     for (int j = 0; j < n; j++) {
-      acc = sky_compress_t<UNROLLED>(acc, w[0]);
+      acc = sky_compress_v(acc, w[0]);
       w[0] = w[1];
       w[1] = w[2];
       w[2] = w[3];
     }
   }
   return acc;
-}
-__device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks) {
-  return sky_hash_message_t<true>(m, chunks);
 }
 
 // Wave-level helpers for task loops whose waves work independently (no
