@@ -41,6 +41,27 @@ def test_python_binding_covers_header():
     assert not missing, missing
 
 
+def test_integration_rust_block_matches_header():
+    """INTEGRATION.md's pkgpu-sys extern block declares every entry point of
+    the header with the same number of parameters."""
+    hdr = open(os.path.join(ROOT, "include", "provekit_gpu.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    c_params = {}
+    for m in re.finditer(r"\b(pkgpu_\w+)\s*\(([^;{]*?)\)\s*;", hdr):
+        args = " ".join(m.group(2).split())
+        c_params[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    a = doc.index('extern "C" {')
+    block = doc[a:doc.index("\n}", a)]
+    rust_params = {}
+    for m in re.finditer(r"pub fn (pkgpu_\w+)\(([^)]*)\)", block):
+        args = " ".join(m.group(2).split())
+        rust_params[m.group(1)] = 0 if not args else args.count(",") + 1
+    assert set(rust_params) == set(c_params), sorted(set(c_params) ^ set(rust_params))
+    bad = {n: (c_params[n], rust_params[n]) for n in c_params if c_params[n] != rust_params[n]}
+    assert not bad, bad
+
+
 def test_abi_version_and_error_string():
     import provekit_amd as pk
     L = pk.lib()
