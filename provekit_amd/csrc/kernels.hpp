@@ -16,6 +16,7 @@ struct NttPassDesc {
   uint64_t chunk;
   uint64_t Q;       // low stride: 2^q * M elements below the window
   int r, jb, canon, tl, bpt;
+  int chunk_pow2;   // chunk is a power of two: g % chunk is a mask
 };
 // Schedule knobs of one transform: tiles of at most 2^tile_log elements,
 // bpt butterflies per thread, earlier passes keep runs of >= 2^min_run

@@ -112,7 +112,15 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
 
   for (int e = tid; e < N; e += T) {
     const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
-    fr_t v = a.src ? fr_load(a.src + (g % a.chunk)) : fr_load(a.data + g);
+    fr_t v;
+    if (a.src) {
+      // a 64-bit remainder is ~50 instructions; the chunk is a power of two
+      // whenever the interleave width is
+      const uint64_t gc = a.chunk_pow2 ? (g & (a.chunk - 1)) : g % a.chunk;
+      v = fr_load(a.src + gc);
+    } else {
+      v = fr_load(a.data + g);
+    }
     lds[e] = v;
   }
   __syncthreads();
@@ -275,6 +283,7 @@ std::vector<NttPassDesc> ntt_plan(Device& d, fr_t* data, uint64_t len, uint64_t 
     a.data = data;
     a.src = (i == 0) ? src : nullptr;
     a.chunk = chunk;
+    a.chunk_pow2 = chunk != 0 && is_pow2(chunk);
     a.rr = rr;
     a.Q = (1ull << (L - plan[i].s0 - plan[i].r)) * M;
     a.r = plan[i].r;
