@@ -35,14 +35,15 @@ HBM_PEAK_GBS = 8000.0
 # VALU: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz; 32-bit integer ops have no
 # packed form, so this is the integer issue peak (lane-instructions/s).
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
-# A Montgomery product of 8x32-bit digits cannot issue fewer than 64 + 64
-# digit multiply-adds (a*b and m*p) plus as many carry adds: 256 VALU
-# instructions, so the multiplier's speed of light is VALU peak / 256.
-MODMUL_PEAK_G = VALU_PEAK_TOPS * 1e3 / 256
+# A Montgomery product of 8x32-bit digits issues 64 + 64 digit multiply-adds
+# (a*b and m*p) and a carry add after each, except the 16 top-limb terms that
+# cannot carry out of the accumulator: 240 VALU instructions, so the
+# multiplier's speed of light is VALU peak / 240.
+MODMUL_PEAK_G = VALU_PEAK_TOPS * 1e3 / 240
 # A Montgomery squaring forms each cross product of a*a once: 36 + 64 digit
-# multiply-adds plus as many carry adds, 200 VALU instructions.  The
+# multiply-adds and 100 - 16 carry adds, 184 VALU instructions.  The
 # Skyscraper kernels run squarings only, so they are priced against this.
-MODSQR_PEAK_G = VALU_PEAK_TOPS * 1e3 / 200
+MODSQR_PEAK_G = VALU_PEAK_TOPS * 1e3 / 184
 
 
 class Hip:
@@ -770,7 +771,7 @@ def main():
         "traffic_algorithmic": 32 * N + 32 * leaves,
         "valu_busy_pmc": load_valu_busy("commit:k_hash_fold"),
         "work_per_launch": f"{compressions_leaf} compressions x 14 Montgomery squarings",
-        "peak_basis": f"VALU int32 issue {VALU_PEAK_TOPS:.1f} T lane-instr/s / 200 instr per 8x32 Montgomery squaring (36 + 64 multiply-adds, as many carry adds)",
+        "peak_basis": f"VALU int32 issue {VALU_PEAK_TOPS:.1f} T lane-instr/s / 184 instr per 8x32 Montgomery squaring (36 + 64 multiply-adds, 84 carry adds)",
         # the same launch seen against HBM: the kernel moves little per
         # instruction, so this fraction stays small by construction
         "hbm_achieved_GBs": (32 * N + 32 * leaves) / (leaf_ms / 1e3) / 1e9,

@@ -258,6 +258,11 @@ __device__ __forceinline__ fr_t fr_sub_canon(const fr_t& a, const fr_t& b) {
 // product latency (1700 -> 1400 cycles measured; the same rate at full
 // occupancy).  The quotient digit -(x + (x << 28)) is asm too: written in C
 // the compiler folds it back into a v_mul_lo_u32.
+// Operand bound: a_7 + b_7 <= 2^32 - 18 on the top limbs, which lazy x lazy
+// (0.76 * 2^32) and canonical x any value below 4p (0.95 * 2^32) meet.  In
+// columns 7-14 the products that hold a top limb open the column and are
+// summed without a carry add (the accumulator cannot pass 2^64 there): 16
+// fewer instructions per product.
 __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   uint32_t m[8];
   fr_t r;
@@ -269,7 +274,7 @@ __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   return r;
 }
 
-// a lazy (any a < 2^255) -> a^2*2^-256 lazy, the value and bytes fr_mul(a, a)
+// a lazy (a < 2p) -> a^2*2^-256 lazy, the value and bytes fr_mul(a, a)
 // gives.  Each cross product a_i a_j (i < j) is formed once, against the
 // doubled limbs of a: d[j] = (a_j << 1) | (a_{j-1} >> 31) and, next to the
 // diagonal, u[j] = a_j << 1 (fr_sqr_asm.inc, written by gen_fr_mul.py, which

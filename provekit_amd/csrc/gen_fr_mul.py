@@ -23,7 +23,9 @@ one v_mul_lo_u32), for k >= 8 the low word is output digit k - 8; then the
 accumulator shifts down one word (in C: register renaming and one move).
 The first add-with-carry of each column writes c2 from zero instead of
 adding into it, which saves a zeroing move per column (15 per product: on a
-lone wave the unrolled Skyscraper compression went 30.3k -> 28.5k cycles)."""
+lone wave the unrolled Skyscraper compression went 30.3k -> 28.5k cycles).
+In columns 7-14 the products that hold a top limb come first and take no
+carry add at all (see no_carry_terms for the operand bound that allows it)."""
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -57,21 +59,50 @@ def column_terms(k, square=False):
     return terms
 
 
+def no_carry_terms(k, square=False):
+    """The terms of column k that are summed first and without a carry add.
+    Column k >= 7 opens with the accumulator below 17 * 2^32 (the previous
+    column's high word plus at most 16 carries).  Product: the two terms with
+    a top limb, a_{k-7} b_7 and a_7 b_{k-7}; their sum stays below 2^64 - 17 *
+    2^32 when top(a) + top(b) <= 2^32 - 18 (lazy times lazy: 0.76 * 2^32;
+    canonical times a value below 4p: 0.95 * 2^32).  Squaring (a < 2p, so
+    the doubled top limb d_7 or u_7 is below 0.76 * 2^32): a_{k-7} d_7 (u_7 in
+    column 13) and m_{k-7} p_7 (p_7 < 0.19 * 2^32).  Column 14 holds only
+    a_7 b_7 (or a_7^2) and m_7 p_7, both small."""
+    if k < 7:
+        return []
+    if square:
+        nc = [t for t in column_terms(k, True)
+              if t[1] in ("d[7]", "u[7]") or (k == 14 and t[0] == "a.l[7]")]
+    else:
+        nc = [t for t in column_terms(k, False)
+              if t[0] == "a.l[7]" or t[1] == "b.l[7]"]
+    nc += [t for t in column_terms(k, square) if t[0] == f"m[{k - 7}]" and t[1] == "p_limb(7)"
+           and (square or k == 14)]
+    return nc
+
+
 def column(k, square=False):
-    terms = column_terms(k, square)
+    nc = no_carry_terms(k, square)
+    carrying = [t for t in column_terms(k, square) if t not in nc]
+    # operands: %0 c01, then %1 c2 and %2 cy, or only %1 cy when nothing carries
+    cy = "%2" if carrying else "%1"
     body, ops = [], []
-    n = 3  # %0 c01, %1 c2, %2 cy
-    for t, (x, y, cls) in enumerate(terms):
-        body.append(f"v_mad_u64_u32 %0, %2, %{n}, %{n + 1}, %0")
-        # the column's first carry starts c2 (0 + 0 + carry): no zeroing move
-        body.append("v_addc_co_u32_e64 %1, %2, 0, 0, %2" if t == 0 else
-                    "v_addc_co_u32_e64 %1, %2, 0, %1, %2")
+    n = 3 if carrying else 2
+    for t, (x, y, cls) in enumerate(nc + carrying):
+        body.append(f"v_mad_u64_u32 %0, {cy}, %{n}, %{n + 1}, %0")
+        if t == len(nc):
+            # the column's first carry starts c2 (0 + 0 + carry): no zeroing move
+            body.append("v_addc_co_u32_e64 %1, %2, 0, 0, %2")
+        elif t > len(nc):
+            body.append("v_addc_co_u32_e64 %1, %2, 0, %1, %2")
         ops += [f'"v"({x})', f'"{cls}"({y})']
         n += 2
+    outs = '"+v"(c01), "=&v"(c2), "=&s"(cy)' if carrying else '"+v"(c01), "=&s"(cy)'
     out = []
     if body:
         text = "\\n\\t".join(body)
-        out.append(f'  asm("{text}"\n               : "+v"(c01), "=&v"(c2), "=&s"(cy)\n'
+        out.append(f'  asm("{text}"\n               : {outs}\n'
                    f'               : {", ".join(ops)});')
     if k < 8:
         # m_k from the column's low word, then m_k * p0 into the column
@@ -81,7 +112,7 @@ def column(k, square=False):
                    f'               : "v"((uint32_t)c01), "s"(p_limb(0)));')
     else:
         out.append(f"  r.l[{k - 8}] = (uint32_t)c01;")
-    out.append("  c01 = (c01 >> 32) | ((uint64_t)c2 << 32);")
+    out.append("  c01 = (c01 >> 32) | ((uint64_t)c2 << 32);" if carrying else "  c01 >>= 32;")
     return out
 
 
@@ -97,10 +128,11 @@ def write(name, head, square):
 
 def main():
     write("fr_mul_asm.inc", "The body of fr_mul (fr.hpp):\n"
-          "// in a, b (lazy); out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there.",
+          "// in a, b (lazy, or one canonical and one < 4p: a_7 + b_7 <= 2^32 - 18);\n"
+          "// out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there.",
           False)
     write("fr_sqr_asm.inc", "The body of fr_sqr (fr.hpp):\n"
-          "// in a (< 2^255) and its doubled limbs u, d; out r = a * a * 2^-256 (lazy);\n"
+          "// in a (< 2p) and its doubled limbs u, d; out r = a * a * 2^-256 (lazy);\n"
           "// m[8], c01, c2, cy declared there.", True)
 
 

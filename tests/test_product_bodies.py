@@ -11,7 +11,10 @@ generator's term lists with Python integers:
   sum to a*a for every a < 2^255, including limbs with their top bit set;
 - running the column schedule with those terms gives a*b*2^-256 mod p
   (closed form), lazy (< 2p) for lazy inputs, and the squaring schedule
-  gives the same 256-bit result as the product schedule with b = a.
+  gives the same 256-bit result as the product schedule with b = a;
+- the terms each column sums without a carry add stay below 2^64 for every
+  operand pair within the stated bound (top limbs summing to at most
+  2^32 - 18), and operands past it are caught dropping a carry.
 
 A wrong index or a dropped doubled limb in the generator fails here without
 a GPU; the GPU tests then compare the kernels that use them with the oracle.
@@ -25,6 +28,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 P = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 M32 = (1 << 32) - 1
+M64 = (1 << 64) - 1
 
 
 @pytest.fixture(scope="module")
@@ -55,25 +59,43 @@ def value(env, expr, m):
     return env[name][idx]
 
 
-def run_schedule(gen, a, b, square):
-    """The column schedule of the generated body, on Python integers."""
+def run_schedule(gen, a, b, square, overflow=None):
+    """The column schedule of the generated body on 32/64-bit registers: c01
+    (64 bits) and c2.  Each column opens with its no-carry terms, whose
+    carry-out the asm drops (a drop is recorded in `overflow`), then the
+    carrying terms count their carries into c2."""
     env = operand_env(a, b)
     p0inv = (-pow(P, -1, 1 << 32)) % (1 << 32)
     m = [0] * 8
-    acc = 0
+    c01 = 0
     out = []
     for k in range(15):
-        for x, y, _ in gen.column_terms(k, square):
-            acc += value(env, x, m) * value(env, y, m)
+        nc = gen.no_carry_terms(k, square)
+        carrying = [t for t in gen.column_terms(k, square) if t not in nc]
+        for x, y, _ in nc:
+            c01 += value(env, x, m) * value(env, y, m)
+            if c01 >> 64:
+                if overflow is None:
+                    raise AssertionError(f"no-carry term overflows in column {k}")
+                overflow.append(k)
+                c01 &= M64
+        c2 = 0
+        for x, y, _ in carrying:
+            c01 += value(env, x, m) * value(env, y, m)
+            c2 += c01 >> 64
+            c01 &= M64
         if k < 8:
-            m[k] = ((acc & M32) * p0inv) & M32
-            acc += m[k] * limbs(P)[0]
-            assert acc & M32 == 0
+            m[k] = ((c01 & M32) * p0inv) & M32
+            c01 += m[k] * limbs(P)[0]
+            c2 += c01 >> 64
+            c01 &= M64
+            assert c01 & M32 == 0
         else:
-            out.append(acc & M32)
-        acc >>= 32
-    out.append(acc & M32)
-    assert acc >> 32 == 0
+            out.append(c01 & M32)
+        assert c2 <= M32
+        c01 = (c01 >> 32) | (c2 << 32)
+    out.append(c01 & M32)
+    assert c01 >> 32 == 0
     return sum(w << (32 * i) for i, w in enumerate(out))
 
 
@@ -116,6 +138,43 @@ def test_product_and_square_schedules(gen):
         s = run_schedule(gen, a, a, True)
         assert s == run_schedule(gen, a, a, False)
         assert s < 2 * P
+
+
+def top_heavy(top):
+    # the largest value with this top limb: every lower limb all ones
+    return (top << 224) | ((1 << 224) - 1)
+
+
+def test_no_carry_terms_within_bound(gen):
+    """Operands at the edge of each case the callers pass: lazy x lazy, a
+    canonical value times one below 4p, and top limbs summing to exactly
+    2^32 - 18, with every lower limb all ones; squarings up to 2p - 1."""
+    rinv = pow(1 << 256, -1, P)
+    top = lambda x: x >> 224
+    cases = [(2 * P - 1, 2 * P - 1), (P - 1, 4 * P - 1), (4 * P - 1, P - 1),
+             (top_heavy(top(2 * P) - 1), top_heavy(top(2 * P) - 1)),
+             (top_heavy(top(P) - 1), top_heavy(top(4 * P) - 1)),
+             (top_heavy((1 << 31) - 9), top_heavy((1 << 31) - 9)),
+             (top_heavy((1 << 32) - 18), top_heavy(0))]
+    for a, b in cases:
+        assert top(a) + top(b) <= (1 << 32) - 18
+        r = run_schedule(gen, a, b, False)
+        assert r % P == a * b * rinv % P and r < 1 << 256
+    for a in (2 * P - 1, top_heavy(top(2 * P) - 1), P - 1):
+        s = run_schedule(gen, a, a, True)
+        assert s == run_schedule(gen, a, a, False) and s % P == a * a * rinv % P
+
+
+def test_no_carry_overflow_is_detected(gen):
+    # operands past the bound (top limbs summing to 2^32 and more) drop a
+    # carry in the no-carry terms: the schedule reports it and the result
+    # is wrong, so the bound above is what keeps the products exact
+    rinv = pow(1 << 256, -1, P)
+    for a, b in [((1 << 256) - 1, (1 << 256) - 1), (top_heavy(1 << 31), top_heavy(1 << 31))]:
+        dropped = []
+        r = run_schedule(gen, a, b, False, overflow=dropped)
+        assert dropped, hex(a)
+        assert r % P != a * b * rinv % P
 
 
 def test_generated_files_are_current(gen, tmp_path, monkeypatch):
