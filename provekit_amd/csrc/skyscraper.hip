@@ -39,27 +39,31 @@ __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out,
   fr_store(out + i, sky_compress(fr_load(in + 2 * i), fr_load(in + 2 * i + 1)));
 }
 
-// The last levels, n_in <= 64 nodes down to the root, in one wave: the level
-// sits in LDS and each step halves the active lanes.  These levels are
+// The last levels, n_in <= 128 nodes down to the root, in one wave: lane t
+// hashes nodes 2t and 2t + 1 of the first of them from memory, then every
+// level takes its two children from neighbouring lanes by shuffles, so no
+// level waits on LDS, a barrier or its own stores.  These levels are
 // latency-bound (one compression per level on the critical path), so a
-// single wave per level is as fast as any spread and needs no launches.
+// single wave is as fast as any spread and needs no launches.
 __global__ __launch_bounds__(64) void k_merkle_tail(fr_t* nodes, uint64_t n_in) {
-  __shared__ fr_t lvl[64];
   const int t = threadIdx.x;
-  if (t < (int)n_in) lvl[t] = fr_load(nodes + t);
-  __syncthreads();
+  bool on = (uint64_t)t < (n_in >> 1);
+  fr_t x = {}, y = {};
+  if (on) {
+    x = fr_load(nodes + 2 * t);
+    y = fr_load(nodes + 2 * t + 1);
+  }
   uint64_t off = n_in;
   for (uint64_t n = n_in; n > 1; n >>= 1) {
-    const uint64_t h = n >> 1;
-    fr_t v;
-    if ((uint64_t)t < h) v = sky_compress(lvl[2 * t], lvl[2 * t + 1]);
-    __syncthreads();
-    if ((uint64_t)t < h) {
-      lvl[t] = v;
+    fr_t v = {};
+    if (on) {
+      v = sky_compress(x, y);
       fr_store(nodes + off + t, v);
     }
-    __syncthreads();
-    off += h;
+    off += n >> 1;
+    x = fr_shfl(v, (2 * t) & 63);
+    y = fr_shfl(v, (2 * t + 1) & 63);
+    on = (uint64_t)t < (n >> 2);
   }
 }
 
@@ -111,7 +115,7 @@ void sky_merkle_dev(const fr_t* leaves, uint64_t n, fr_t* nodes, hipStream_t s) 
     PK_HIP(hipGetLastError());
   }
   fr_t* lvl = nodes;
-  while (n > 64) {
+  while (n > 128) {
     // one-wave workgroups: the narrow upper levels spread over CUs instead of
     // sharing SIMDs
     const unsigned t = n / 2 >= (1u << 16) ? 256 : 64;
