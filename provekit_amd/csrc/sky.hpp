@@ -107,11 +107,11 @@ __device__ __forceinline__ void sky_double_round(fr_t& l, fr_t& r, int dr) {
 }
 
 // compress(l, r) for arbitrary 256-bit inputs; canonical output.  The nine
-// double rounds are straight-line code (~5.6k instructions, ~34 KB): no
+// double rounds are straight-line code (~5.0k instructions, ~34 KB): no
 // branches, the round constants as literals, and the scheduler works across
 // rounds.  On one wave this took 30.3k cycles against 35.3k for a loop over
-// the double rounds (22.7k now, with fr_add_ilv and fr_sqr), and the
-// full-occupancy rate is ~2 % higher.  Straight-line code starting cold costs
+// the double rounds (21.7k now, with fr_add_ilv, fr_sqr and the top-limb
+// terms), and the full-occupancy rate is ~2 % higher.  Straight-line code starting cold costs
 // little (~0.3 us per launch, measured), but several inlined copies in a
 // kernel whose waves all stream through them would overflow the 64 KB
 // instruction cache: every kernel here calls it from one place.  Two entry

@@ -2,7 +2,7 @@
 // left fold (SkyscraperHashEngine::hash_many,
 // provekit/common/src/skyscraper/whir.rs:42-100) and binary Merkle trees.
 //
-// Every compression is ~5.2k VALU instructions (14 Montgomery squarings, 4
+// Every compression is ~5.0k VALU instructions (14 Montgomery squarings, 4
 // byte-S-box layers, lazy reductions), so these kernels are VALU-bound; one
 // lane owns one message, which keeps every lane busy with no cross-lane
 // traffic.  The upper tree levels are latency-bound (a compression is a long
