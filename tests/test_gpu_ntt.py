@@ -99,3 +99,41 @@ def test_rs_points_generator_next_order(gpu, oracle):
         assert np.array_equal(rs.evaluation_points(0, cw, idx), oracle.rs_evaluation_points(cw, idx))
         assert np.array_equal(rs.generator(cw), oracle.root_of_unity(cw))
     assert rs.next_order(1000) == 1024 and rs.next_order((1 << 28) + 1) is None
+
+
+def _bitrev(i, bits):
+    return int(format(i, f"0{bits}b")[::-1], 2)
+
+
+@pytest.mark.parametrize("k", [26, 28])
+def test_ntt_max_orders_closed_form_and_round_trip(gpu, k):
+    """Orders up to the largest RSFr::next_order accepts (2^28): one nonzero
+    coefficient c at position j transforms to out[i] = c * w^(j * bitrev(i))
+    (the closed form of ntt_nr, checked at sampled positions), and intt_rn
+    brings back exactly the delta.  2^26 also round-trips a dense vector."""
+    from tests.util import P, to_int
+    n = 1 << k
+    w = pow(pow(5, (P - 1) >> 28, P), (1 << 28) // n, P)
+    rng = np.random.default_rng(k)
+    j = int(rng.integers(1, n))
+    c = 0x1234567890ABCDEF1122334455667788 % P  # stored limbs, canonical
+    a = np.zeros((n, 4), dtype=np.uint64)
+    a[j] = [(c >> (64 * q)) & 0xFFFFFFFFFFFFFFFF for q in range(4)]
+    say(f"ntt_nr 2^{k} delta at {j}")
+    gpu.ntt_nr(a, n, 1)
+    picks = [0, 1, 2, n // 2, n - 1] + [int(x) for x in rng.integers(0, n, 48)]
+    for i in picks:
+        assert to_int(a[i]) == c * pow(w, j * _bitrev(i, k), P) % P, i
+    say(f"intt_rn 2^{k}")
+    gpu.intt_rn(a)
+    nz = np.flatnonzero(a.any(axis=1))
+    assert nz.tolist() == [j] and to_int(a[j]) == c
+    del a
+    if k <= 26:
+        v = rng.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+        v[:, 3] &= np.uint64((1 << 61) - 1)  # < 2^253 < p
+        b = v.copy()
+        say(f"ntt_nr + intt_rn 2^{k} dense")
+        gpu.ntt_nr(b, n, 1)
+        gpu.intt_rn(b)
+        assert np.array_equal(b, v)
