@@ -71,6 +71,34 @@ def test_merkle_device(gpu, oracle, leaves_log):
     assert np.array_equal(got, exp)
 
 
+def test_merkle_device_2p26_sampled_paths(gpu, oracle):
+    """A 2^26-leaf tree (4 GiB of nodes, 64-bit node offsets): the leaves are
+    copied through, and along the paths of sampled leaves every node is the
+    oracle's compression of its two children, up to the root."""
+    k = 26
+    n = 1 << k
+    rng = np.random.default_rng(26)
+    leaves = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    say(f"merkle n=2^{k}")
+    d_nodes = gpu.DeviceBuffer(32 * (2 * n - 1))
+    d_leaves = gpu.DeviceBuffer.from_array(leaves)
+    gpu.merkle_device(d_leaves, n, d_nodes)
+    nodes = d_nodes.download(np.uint64).reshape(2 * n - 1, 4)
+    d_nodes.free()
+    d_leaves.free()
+    assert np.array_equal(nodes[:n], leaves)
+    picks = [0, n - 1] + [int(x) for x in rng.integers(0, n, 14)]
+    for leaf in picks:
+        i, off, width = leaf, 0, n
+        while width > 1:
+            i >>= 1
+            up = off + width
+            exp = oracle.sky_compress(nodes[off + 2 * i], nodes[off + 2 * i + 1])
+            assert np.array_equal(nodes[up + i], exp), (leaf, width)
+            off, width = up, width >> 1
+    assert off == 2 * n - 2
+
+
 @pytest.mark.parametrize("log_n,blowup,coset", [(4, 4, True), (8, 2, False), (12, 4, True), (12, 4, False),
                                                  (14, 2, True)])
 def test_commit_small(gpu, oracle, log_n, blowup, coset):
