@@ -112,6 +112,32 @@ def test_gpu_merkle(gpu, oracle, log_n):
 
 
 @pytest.mark.gpu
+def test_gpu_merkle_2p24_sampled_paths(gpu, oracle):
+    """The 2^24-leaf Poseidon2 tree of the benchmark workload: along the
+    paths of sampled leaves every node is the oracle's two-element hash of its
+    children, up to the returned root."""
+    import numpy as np
+    k = 24
+    n = 1 << k
+    rng = np.random.default_rng(24)
+    leaves = rng.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+    leaves[:, 3] &= np.uint64((1 << 61) - 1)  # < 2^253 < p: canonical
+    print(f"[case] poseidon2 merkle 2^{k}", flush=True)
+    root, nodes = gpu.poseidon2_merkle(leaves)
+    assert np.array_equal(nodes[:n], leaves)
+    picks = [0, n - 1] + [int(x) for x in rng.integers(0, n, 14)]
+    for leaf in picks:
+        i, off, width = leaf, 0, n
+        while width > 1:
+            i >>= 1
+            up = off + width
+            exp = oracle.p2_hash(nodes[off + 2 * i:off + 2 * i + 2])
+            assert np.array_equal(nodes[up + i], exp), (leaf, width)
+            off, width = up, width >> 1
+    assert np.array_equal(root, nodes[-1])
+
+
+@pytest.mark.gpu
 def test_gpu_merkle_graph_replays(gpu, oracle):
     # the captured graph, replayed several times with eager work in between,
     # gives the eager result every time
