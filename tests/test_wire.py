@@ -214,6 +214,11 @@ def test_gpu_rejects_malformed(gpu, name):
 def test_gpu_rejects_bad_interner(gpu):
     with pytest.raises(gpu.ProvekitGpuError):
         gpu.decode_interner(encode_interner([P]))
+    # the largest 256-bit value among valid ones: rejected, and the product
+    # that converts it is fed a zero in its place
+    with pytest.raises(gpu.ProvekitGpuError):
+        gpu.decode_interner(encode_interner([1, (1 << 256) - 1, P - 1]))
+    assert [unmont(x) for x in gpu.decode_interner(encode_interner([1, P - 1]))] == [1, P - 1]
 
 
 def encode_matrix_np(rs, cs, ids, rows, cols) -> bytes:
