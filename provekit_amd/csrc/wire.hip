@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_to_mont(const fr_t* raw, uint64_t n, fr
   if (!br) atomicOr(err, 1u);  // x >= p
   // a rejected x (its output is never used) enters the product as 0: fr_mul
   // needs operands within its bound on the two top limbs, and x >= p may not be
-  fr_t xs = x;
+  fr_t xs;
 #pragma unroll
   for (int k = 0; k < 8; k++) xs.l[k] = br ? x.l[k] : 0u;
   fr_t m = fr_mul(xs, r2);
