@@ -362,3 +362,26 @@ def test_gpu_sumcheck_prove(gpu, oracle, m, mode):
     assert np.array_equal(got["coeffs"], exp["coeffs"])
     assert np.array_equal(got["alphas"], exp["alphas"])
     assert np.array_equal(got["final"], exp["final"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m", [20, 22])
+def test_gpu_sumcheck_prove_large(gpu, oracle, m):
+    """The whole plain sumcheck at 2^20 and 2^22 constraints (the grid rounds,
+    the claim of round 0 and the one-workgroup tail at full size), bit-exact
+    against the oracle: eq point, every round polynomial, every challenge and
+    the final folded values."""
+    n = 1 << m
+    rng = np.random.default_rng(m)
+
+    def canon(k):
+        x = rng.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+        x[:, 3] &= np.uint64((1 << 61) - 1)  # < 2^253 < p
+        return x
+    a, b, c = canon(0), canon(1), canon(2)
+    print(f"[case] sumcheck m={m} plain (large)", flush=True)
+    got = gpu.sumcheck_prove(a, b, c, m, blinding=None, seed=91)
+    exp = oracle.sumcheck_prove(a, b, c, m, None, seed=91)
+    assert exp["ok"]
+    for key in ("r", "coeffs", "alphas", "final"):
+        assert np.array_equal(got[key], exp[key]), key
