@@ -385,3 +385,18 @@ def test_gpu_sumcheck_prove_large(gpu, oracle, m):
     assert exp["ok"]
     for key in ("r", "coeffs", "alphas", "final"):
         assert np.array_equal(got[key], exp[key]), key
+
+
+@pytest.mark.gpu
+def test_gpu_sparse_mul_2p22_rows(gpu, oracle):
+    """HydratedSparseMatrix * z at 2^22 rows (1-4 entries per row, the
+    benchmark's constraint count), bit-exact against the oracle."""
+    from tests.test_transpose import random_csr as random_csr_np
+    rows = cols = 1 << 22
+    rng = np.random.default_rng(22)
+    rs, cs, vs = random_csr_np(rng, rows, cols, seed=22)
+    z = rng.integers(0, 1 << 63, size=(cols, 4), dtype=np.uint64)
+    z[:, 3] &= np.uint64((1 << 61) - 1)  # < 2^253 < p
+    print(f"[case] spmv 2^22 rows nnz={cs.shape[0]}", flush=True)
+    A = gpu.SparseMatrix(rows, cols, rs, cs, vs)
+    assert np.array_equal(A * z, oracle.sparse_mul(rows, cols, rs, cs, vs, z))
