@@ -277,17 +277,26 @@ int pkgpu_eq_evaluations_device(const uint64_t* r, size_t m, size_t num_entries,
 int pkgpu_sumcheck_r1cs_round_device(uint64_t* a, uint64_t* b, uint64_t* c, uint64_t* eq,
                                      size_t size, const uint64_t* fold, uint64_t* out,
                                      void* stream);
-/* The whole sumcheck of run_zk_sumcheck_prover on the device, with this
+/* The whole sumcheck of run_zk_sumcheck_prover
+ * (provekit/prover/src/whir_r1cs.rs:519-621) on the device, with this
  * library's Skyscraper transcript (seeded) in place of spongefish.
- * a, b, c: device, 2^m elements each, consumed.  blinding: host m x 4 Fr
- * (reference ZK mode: sum_g absorbed, rho squeezed, claim rho * sum_g) or
- * NULL (plain mode: rho = 0, claim = sum eq (a b - c) computed on device).
+ * a, b, c: device, 2^m elements each, consumed (m = 0: one element each, run
+ * zero-padded to two like pad_to_pow2_len_min2).  blinding: host m x 4 Fr
+ * (reference ZK mode: sum_g absorbed, rho squeezed, claim rho * sum_g, and
+ * after the last round blinding_eval = sum_i g_i(alpha_i) absorbed, :613-618;
+ * the univariates are the 4m elements the commit wrote at w1[offset..], so
+ * this is <expand_powers::<4>(alpha), w1[offset..offset+4m]>; needs m >= 1)
+ * or NULL (plain mode: rho = 0, claim = sum eq (a b - c) computed on device,
+ * blinding_eval = 0 and not absorbed).
  * Outputs (host): coeffs m x 4 Fr, alphas m Fr, r m Fr (the eq point),
- * final_claim 1 Fr.  Returns PKGPU_ERR_INTERNAL if a round identity fails. */
+ * final_claim 1 Fr (the last cubic at alpha_{m-1}), rho and blinding_eval 1
+ * Fr each (may be NULL).  The verifier's check is final_claim - rho *
+ * blinding_eval = eq(r, alpha)(a(alpha) b(alpha) - c(alpha)).  Returns
+ * PKGPU_ERR_INTERNAL if a round identity fails. */
 int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
                                 const uint64_t* blinding, uint64_t seed, uint64_t* coeffs,
                                 uint64_t* alphas, uint64_t* r, uint64_t* final_claim,
-                                void* stream);
+                                uint64_t* rho, uint64_t* blinding_eval, void* stream);
 
 /* ------------------------------------------------------------ commit */
 /* WHIR-style commitment of one polynomial of n coefficients (n, blowup powers

@@ -83,7 +83,8 @@ void or_sumcheck_r1cs_round(ofr* a, ofr* b, ofr* c, ofr* eq, size_t size, const 
                             ofr out[3]);
 ofr or_eval_cubic(const ofr poly[4], ofr x);
 int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uint64_t seed,
-                      ofr* coeffs, ofr* alphas, ofr* r_out, ofr* final_claim);
+                      ofr* coeffs, ofr* alphas, ofr* r_out, ofr* final_claim, ofr* extra);
+void or_sumcheck_plant_fault(int kind);
 void or_transcript_run(uint64_t seed, const ofr* absorbs, size_t n_abs, ofr* squeezes,
                        size_t n_sq);
 

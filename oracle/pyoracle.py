@@ -74,7 +74,8 @@ def lib():
             "or_r1cs_decode": ([_vp, _sz, _vp, _vp, _vp, _vp, _vp], _i),
             "or_sumcheck_r1cs_round": ([_vp, _vp, _vp, _vp, _sz, _vp, _vp], None),
             "or_eval_cubic": ([_vp, OFr], OFr),
-            "or_sumcheck_prove": ([_vp, _vp, _vp, _sz, _vp, _u64, _vp, _vp, _vp, _vp], _i),
+            "or_sumcheck_prove": ([_vp, _vp, _vp, _sz, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),
+            "or_sumcheck_plant_fault": ([_i], None),
             "or_transcript_run": ([_u64, _vp, _sz, _vp, _sz], None),
             "or_fr_mul": ([OFr, OFr], OFr),
             "or_fr_add": ([OFr, OFr], OFr),
@@ -438,16 +439,31 @@ def eval_cubic(poly, x) -> np.ndarray:
 
 def sumcheck_prove(a, b, c, m: int, blinding=None, seed: int = 0):
     """Full sumcheck (oracle/transcript.c).  Returns dict with coeffs (m,4,4),
-    alphas (m,4), r (m,4), final (4,), ok (bool)."""
+    alphas (m,4), r (m,4), final (4,), rho, sum_g, blinding_eval, state (4,)
+    each, ok (bool).  m = 0 pads a, b, c with a zero to 2 elements
+    (pad_to_pow2_len_min2)."""
     A, B, C = (_a(x).reshape(-1, 4).copy() for x in (a, b, c))
-    coeffs = np.zeros((m, 4, 4), np.uint64)
-    alphas = np.zeros((m, 4), np.uint64)
-    rr = np.zeros((m, 4), np.uint64)
+    if m == 0:
+        A, B, C = (np.concatenate([x[:1], np.zeros((1, 4), np.uint64)]) for x in (A, B, C))
+    coeffs = np.zeros((max(m, 1), 4, 4), np.uint64)
+    alphas = np.zeros((max(m, 1), 4), np.uint64)
+    rr = np.zeros((max(m, 1), 4), np.uint64)
     fin = np.zeros(4, np.uint64)
+    extra = np.zeros((4, 4), np.uint64)
     g = _a(blinding).reshape(-1, 4) if blinding is not None else None
     rc = lib().or_sumcheck_prove(_p(A), _p(B), _p(C), m, _p(g), seed, _p(coeffs), _p(alphas),
-                                 _p(rr), _p(fin))
-    return {"coeffs": coeffs, "alphas": alphas, "r": rr, "final": fin, "ok": rc == 0}
+                                 _p(rr), _p(fin), _p(extra))
+    if rc == -2:
+        raise ValueError("blinding polynomial needs m >= 1")
+    return {"coeffs": coeffs[:m], "alphas": alphas[:m], "r": rr[:m], "final": fin,
+            "rho": extra[0], "sum_g": extra[1], "blinding_eval": extra[2], "state": extra[3],
+            "ok": rc == 0}
+
+
+def plant_fault(kind: int) -> None:
+    """Plant a mistake in the oracle's ZK sumcheck (0 removes it); for the
+    tests that show the verifier's checks catch it."""
+    lib().or_sumcheck_plant_fault(kind)
 
 
 def transcript_run(seed: int, absorbs, n_squeeze: int) -> np.ndarray:

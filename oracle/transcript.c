@@ -52,6 +52,14 @@ static void tr_squeeze(or_transcript* t, size_t k, ofr* out) {
   or_sky_compress(t->s, z, t->s);
 }
 
+/* Planted mistakes for the tests that show the verifier's checks catch them
+ * (0 = none): 1 prefix multiplier doubled once too often, 2 suffix
+ * multiplier equal to the prefix multiplier (not half of it), 3 rho * g
+ * subtracted from c0 instead of added, 4 blinding_eval without the last
+ * univariate.  Test infrastructure: set only by tests/test_sumcheck.py. */
+static int g_fault;
+void or_sumcheck_plant_fault(int kind) { g_fault = kind; }
+
 /* compute_blinding_coefficients_for_round (whir_r1cs.rs:427-492) */
 static void blinding_coeffs(const ofr* g, size_t n, size_t compute_for, const ofr* alphas,
                             ofr out[4]) {
@@ -68,8 +76,9 @@ static void blinding_coeffs(const ofr* g, size_t n, size_t compute_for, const of
     suffix = ofr_add(suffix, ofr_add(or_eval_cubic(g + 4 * i, zero), or_eval_cubic(g + 4 * i, OFR_ONE)));
   ofr pm = OFR_ONE;
   for (size_t i = 0; i < n - 1 - compute_for; i++) pm = ofr_add(pm, pm);
+  if (g_fault == 1) pm = ofr_add(pm, pm);
   ofr two = ofr_add(OFR_ONE, OFR_ONE);
-  ofr sm = ofr_mul(pm, ofr_inv(two));
+  ofr sm = g_fault == 2 ? pm : ofr_mul(pm, ofr_inv(two));
   ofr other = ofr_add(ofr_mul(pm, prefix), ofr_mul(sm, suffix));
   const ofr* cur = g + 4 * compute_for;
   ofr c[4] = {ofr_add(ofr_mul(pm, cur[0]), other), ofr_mul(pm, cur[1]), ofr_mul(pm, cur[2]),
@@ -90,16 +99,40 @@ static ofr sum_over_hypercube(const ofr* g, size_t n) {
   return ofr_add(or_eval_cubic(c, zero), or_eval_cubic(c, OFR_ONE));
 }
 
+/* The blinding evaluation of run_zk_sumcheck_prover (whir_r1cs.rs:613-618):
+ * <expand_powers::<4>(alpha), w1[offset .. offset + 4m]>
+ * (provekit/common/src/prefix_covector.rs:156-166), where the commit wrote
+ * the univariates g (m x 4, row-major) at w1[offset..] (whir_r1cs.rs:118-124),
+ * so the dot product is sum_i sum_j alpha_i^j g_i[j]. */
+static ofr blinding_eval(const ofr* g, size_t m, const ofr* alphas) {
+  ofr acc = {{0}};
+  size_t n = g_fault == 4 && m ? m - 1 : m;
+  for (size_t i = 0; i < n; i++) {
+    ofr pw = OFR_ONE;
+    for (int j = 0; j < 4; j++) {
+      acc = ofr_add(acc, ofr_mul(pw, g[4 * i + j]));
+      pw = ofr_mul(pw, alphas[i]);
+    }
+  }
+  return acc;
+}
+
 /* Full sumcheck over eq(r, x) * (a(x) b(x) - c(x)), size = 2^m.
  * blinding != NULL: reference mode (whir_r1cs.rs:519-621): absorb sum_g,
- *   squeeze rho, claimed sum rho * sum_g, blinding g (m x 4) mixed in.
+ *   squeeze rho, claimed sum rho * sum_g, blinding g (m x 4) mixed in; after
+ *   the last round blinding_eval is computed and absorbed.  Needs m >= 1 (the
+ *   reference's compute_for = n - 1 underflows at m = 0).
  * blinding == NULL: plain mode: rho = 0, the claimed sum is computed as
- *   sum_x eq(r,x)(a b - c) and absorbed in place of sum_g.
- * a, b, c are consumed (folded in place).  coeffs: m x 4, alphas: m,
+ *   sum_x eq(r,x)(a b - c) and absorbed in place of sum_g; no blinding_eval.
+ * a, b, c are consumed (folded in place; 2 elements each when m = 0, the
+ *   reference's pad_to_pow2_len_min2).  coeffs: m x 4, alphas: m,
  * r_out: m (the eq point), final_claim: the last cubic at alpha_{m-1}.
- * Returns 0, or -1 if the round identity c0+c0+c1+c2+c3 = S fails. */
+ * extra (may be NULL): [rho, sum_g, blinding_eval, transcript state at the end].
+ * Returns 0, -1 if the round identity c0+c0+c1+c2+c3 = S fails, -2 for a
+ * blinding polynomial with m = 0. */
 int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uint64_t seed,
-                      ofr* coeffs, ofr* alphas, ofr* r_out, ofr* final_claim) {
+                      ofr* coeffs, ofr* alphas, ofr* r_out, ofr* final_claim, ofr* extra) {
+  if (blinding && m == 0) return -2;
   size_t size = (size_t)1 << m;
   if (size < 2) size = 2;
   or_transcript tr;
@@ -108,9 +141,9 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
   ofr* eq = (ofr*)calloc(size, sizeof(ofr));
   or_eq_evals(r_out, m, (size_t)1 << m, eq);
   ofr zero = {{0}};
-  ofr rho = zero, S;
+  ofr rho = zero, sg = zero, S;
   if (blinding) {
-    ofr sg = sum_over_hypercube(blinding, m);
+    sg = sum_over_hypercube(blinding, m);
     tr_absorb1(&tr, sg);
     tr_squeeze(&tr, 1, &rho);
     S = ofr_mul(rho, sg);
@@ -131,7 +164,7 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
     ofr g[4] = {zero, zero, zero, zero};
     if (blinding) blinding_coeffs(blinding, m, idx, alphas, g);
     ofr* co = coeffs + 4 * idx;
-    co[0] = ofr_add(h[0], ofr_mul(rho, g[0]));
+    co[0] = g_fault == 3 ? ofr_sub(h[0], ofr_mul(rho, g[0])) : ofr_add(h[0], ofr_mul(rho, g[0]));
     ofr g_m1 = ofr_sub(ofr_add(ofr_sub(g[0], g[1]), g[2]), g[3]);
     ofr at_m1 = ofr_add(h[1], ofr_mul(rho, g_m1));
     ofr three_c0 = ofr_add(ofr_add(co[0], co[0]), co[0]);
@@ -145,6 +178,17 @@ int or_sumcheck_prove(ofr* a, ofr* b, ofr* c, size_t m, const ofr* blinding, uin
     S = or_eval_cubic(co, alphas[idx]);
   }
   *final_claim = S;
+  ofr be = zero;
+  if (blinding) {
+    be = blinding_eval(blinding, m, alphas);
+    tr_absorb1(&tr, be);
+  }
+  if (extra) {
+    extra[0] = rho;
+    extra[1] = sg;
+    extra[2] = be;
+    memcpy(extra[3].l, tr.s, sizeof tr.s);
+  }
   free(eq);
   return ok;
 }

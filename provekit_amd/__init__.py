@@ -93,7 +93,7 @@ SIGNATURES = {
     "pkgpu_eq_evaluations": [_vp, _sz, _sz, _vp],
     "pkgpu_eq_evaluations_device": [_vp, _sz, _sz, _vp, _vp],
     "pkgpu_sumcheck_r1cs_round_device": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp],
-    "pkgpu_sumcheck_prove_device": [_vp, _vp, _vp, _sz, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sumcheck_prove_device": [_vp, _vp, _vp, _sz, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_ntt_autotune": [_i, _i, _i, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)],
     "pkgpu_ntt_tuning_load": [ctypes.c_char_p],
     "pkgpu_ntt_tuning_get": [_i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
@@ -602,7 +602,8 @@ def sumcheck_r1cs_round(a: "DeviceBuffer", b: "DeviceBuffer", c: "DeviceBuffer",
 
 def sumcheck_prove(a, b, c, m: int, blinding=None, seed: int = 0, stream=None) -> dict:
     """Whole sumcheck on the device (pkgpu_sumcheck_prove_device).  a, b, c are
-    host (2^m, 4) arrays (copied to the device) or DeviceBuffers (consumed)."""
+    host (2^m, 4) arrays (copied to the device) or DeviceBuffers (consumed).
+    Returns coeffs, alphas, r, final, rho and blinding_eval."""
     _ensure_init()
     bufs = []
     for x in (a, b, c):
@@ -611,11 +612,14 @@ def sumcheck_prove(a, b, c, m: int, blinding=None, seed: int = 0, stream=None) -
     alphas = np.zeros((max(m, 1), 4), np.uint64)
     r = np.zeros((max(m, 1), 4), np.uint64)
     fin = np.zeros(4, np.uint64)
+    rho = np.zeros(4, np.uint64)
+    beval = np.zeros(4, np.uint64)
     g = np.ascontiguousarray(blinding, dtype=np.uint64).reshape(-1, 4) if blinding is not None else None
     _call("pkgpu_sumcheck_prove_device", bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, m,
           _ptr(g) if g is not None else None, seed, _ptr(coeffs), _ptr(alphas), _ptr(r), _ptr(fin),
-          stream)
-    return {"coeffs": coeffs[:m], "alphas": alphas[:m], "r": r[:m], "final": fin}
+          _ptr(rho), _ptr(beval), stream)
+    return {"coeffs": coeffs[:m], "alphas": alphas[:m], "r": r[:m], "final": fin, "rho": rho,
+            "blinding_eval": beval}
 
 
 # ----------------------------------------------------------------- commit

@@ -877,21 +877,39 @@ int pkgpu_sumcheck_r1cs_round_device(uint64_t* a, uint64_t* b, uint64_t* c, uint
 int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
                                 const uint64_t* blinding, uint64_t seed, uint64_t* coeffs,
                                 uint64_t* alphas, uint64_t* r, uint64_t* final_claim,
-                                void* stream) {
+                                uint64_t* rho, uint64_t* blinding_eval, void* stream) {
   return guarded([&] {
     REQ(a && b && c && final_claim && (m == 0 || (coeffs && alphas && r)), pk::PK_ERR_ARG,
         "sumcheck: null pointer");
     REQ(m <= (size_t)pk::kMaxSumcheckVars, pk::PK_ERR_SIZE, "sumcheck: more than 28 variables");
+    // compute_blinding_coefficients_for_round's compute_for = n - 1 underflows
+    // for an empty blinding polynomial (whir_r1cs.rs:437-439)
+    REQ(!(blinding && m == 0), pk::PK_ERR_SIZE, "attempt to subtract with overflow");
     hipStream_t s = stream_or_thread(stream);
     const size_t size = m ? (size_t(1) << m) : 2;
     pk::ScState hst;
     pk::sc_state_init_host(hst, (int)m, blinding ? 1 : 0, reinterpret_cast<const fr_t*>(blinding));
     Staged dst(sizeof(pk::ScState), s), deq(sizeof(fr_t) * size, s),
-        work(sizeof(fr_t) * (pk::sc_partials_elems() + 3), s);
+        work(sizeof(fr_t) * (pk::sc_partials_elems() + 3), s),
+        pad(m ? 0 : 6 * sizeof(fr_t), s);
+    fr_t* da = reinterpret_cast<fr_t*>(a);
+    fr_t* db = reinterpret_cast<fr_t*>(b);
+    fr_t* dc = reinterpret_cast<fr_t*>(c);
+    if (m == 0) {
+      // pad_to_pow2_len_min2: the caller's MLEs hold one element each; the
+      // rounds read two, so run on zero-padded copies
+      fr_t* p = pad.get<fr_t>();
+      pk::zero_dev(p, 6, s);
+      fr_t* src[3] = {da, db, dc};
+      for (int k = 0; k < 3; k++)
+        PK_HIP(hipMemcpyAsync(p + 2 * k, src[k], sizeof(fr_t), hipMemcpyDeviceToDevice, s));
+      da = p;
+      db = p + 2;
+      dc = p + 4;
+    }
     h2d(dst.get<void>(), &hst, sizeof hst, s);
-    pk::sc_prove_dev(dst.get<pk::ScState>(), reinterpret_cast<fr_t*>(a), reinterpret_cast<fr_t*>(b),
-                     reinterpret_cast<fr_t*>(c), deq.get<fr_t>(), (int)m, blinding ? 1 : 0, seed,
-                     work.get<fr_t>(), s);
+    pk::sc_prove_dev(dst.get<pk::ScState>(), da, db, dc, deq.get<fr_t>(), (int)m,
+                     blinding ? 1 : 0, seed, work.get<fr_t>(), s);
     d2h(&hst, dst.get<void>(), sizeof hst, s);
     sync(s);
     if (m) {
@@ -899,8 +917,9 @@ int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
       memcpy(alphas, hst.alpha, sizeof(fr_t) * m);
       memcpy(r, hst.r, sizeof(fr_t) * m);
     }
-    fr_t fin = hst.S;
-    memcpy(final_claim, &fin, 32);
+    memcpy(final_claim, &hst.S, 32);
+    if (rho) memcpy(rho, &hst.rho, 32);
+    if (blinding_eval) memcpy(blinding_eval, &hst.blinding_eval, 32);
     REQ(!hst.error, pk::PK_ERR_INTERNAL,
         "assertion `left == right` failed: sumcheck round identity c0 + c0 + c1 + c2 + c3 = S");
   });

@@ -485,6 +485,21 @@ __global__ __launch_bounds__(64) void k_sc_claim_finalize(ScState* st, const fr_
   }
 }
 
+// After the last round of the reference mode: blinding_eval =
+// <expand_powers::<4>(alpha), w1[offset .. offset + 4m]> (whir_r1cs.rs:613-618,
+// prefix_covector.rs:156-166).  The commit placed the univariates g at
+// w1[offset..], so this is sum_i g_i(alpha_i); it is absorbed like the
+// reference's prover_message.  One lane: m cubics and one compression.
+__global__ __launch_bounds__(64) void k_sc_blinding_eval(ScState* st) {
+  if (threadIdx.x != 0) return;
+  const int m = st->m;
+  fr_t be = fr_zero();
+  for (int i = 0; i < m; i++) be = fr_add(be, cubic(st->g[i], st->alpha[i]));
+  fr_canon(be);
+  st->blinding_eval = be;
+  st->state = sky_compress(st->state, be);
+}
+
 // What thread 0 of a finalizing workgroup reads from ScState: loaded once at
 // kernel start, next to the partial sums, instead of after them (a lone
 // wave would wait out each load), and carried in registers across the rounds
@@ -767,11 +782,15 @@ void eq_table_dev(const fr_t* r, int m, uint64_t num_entries, fr_t* out, hipStre
     PK_HIP(hipGetLastError());
     return;
   }
+  // eq(r, x) = eq(r_hi, x_hi) eq(r_lo, x_lo); only the half-table entries
+  // that x < num_entries reaches are built, so a large m with few entries
+  // (the reference's truncation) stays small
   const int lo_bits = m / 2, hi_bits = m - lo_bits;
-  Scratch hi(sizeof(fr_t) << hi_bits, s), lo(sizeof(fr_t) << lo_bits, s);
-  k_eq_direct<<<nblk(1ull << hi_bits, 256), 256, 0, s>>>(r, hi_bits, 1ull << hi_bits, hi.as<fr_t>(), one);
-  k_eq_direct<<<nblk(1ull << lo_bits, 256), 256, 0, s>>>(r + hi_bits, lo_bits, 1ull << lo_bits,
-                                                         lo.as<fr_t>(), one);
+  const uint64_t lo_n = std::min<uint64_t>(num_entries, 1ull << lo_bits);
+  const uint64_t hi_n = ((num_entries - 1) >> lo_bits) + 1;
+  Scratch hi(sizeof(fr_t) * hi_n, s), lo(sizeof(fr_t) * lo_n, s);
+  k_eq_direct<<<nblk(hi_n, 256), 256, 0, s>>>(r, hi_bits, hi_n, hi.as<fr_t>(), one);
+  k_eq_direct<<<nblk(lo_n, 256), 256, 0, s>>>(r + hi_bits, lo_bits, lo_n, lo.as<fr_t>(), one);
   k_eq_combine<<<nblk(num_entries, 256), 256, 0, s>>>(hi.as<fr_t>(), lo.as<fr_t>(), lo_bits,
                                                       num_entries, out);
   PK_HIP(hipGetLastError());
@@ -834,6 +853,10 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
       k_sc_finalize<3><<<1, kFinThreads, 0, s>>>(st, partials, nb, idx);
     }
     if (fold) cur /= 2;
+    PK_HIP(hipGetLastError());
+  }
+  if (mode != 0 && m >= 1) {
+    k_sc_blinding_eval<<<1, 64, 0, s>>>(st);
     PK_HIP(hipGetLastError());
   }
 }

@@ -25,6 +25,7 @@ struct ScState {
   fr_t S;                                  // current claim
   fr_t rho;                                // blinding challenge (0 in plain mode)
   fr_t sum_g;                              // sum of the blinding polynomial over the cube
+  fr_t blinding_eval;                      // sum_i g_i(alpha_i), after the last round
   fr_t state;                              // transcript state
   fr_t one, half;                          // Montgomery 1 and 1/2
   fr_t r[kMaxSumcheckVars];                // eq point

@@ -150,16 +150,105 @@ def test_plain_sumcheck_verifier_equation(oracle, m):
     assert unmont(out["final"]) == fa
 
 
-def test_reference_mode_rounds_consistent(oracle):
-    m = 5
+def zk_verify(out, m, a, b, c, g):
+    """The reference verifier on a ZK-mode transcript (run_sumcheck_verifier,
+    provekit/verifier/src/whir_r1cs.rs:266-312, and the final check at
+    :249-257), with Python integers: the round sums start at rho * sum_g and
+    hit every cubic's h(0) + h(1); f(alpha) = S_final - rho * blinding_eval
+    must equal eq(r, alpha) (a(alpha) b(alpha) - c(alpha)).  Also the values
+    the prover sends: sum_g = 2^(m-1) sum_i (g_i(0) + g_i(1)) (the sum of
+    sum_i g_i(x_i) over the cube) and blinding_eval = sum_i g_i(alpha_i) (the
+    dot product with the committed univariates, prefix_covector.rs:156-166).
+    Returns the list of the checks that fail."""
+    bad = []
+    r, alpha = ints(out["r"]), ints(out["alphas"])
+    rho, sum_g, beval = unmont(out["rho"]), unmont(out["sum_g"]), unmont(out["blinding_eval"])
+    gi = [g[4 * i: 4 * i + 4] for i in range(m)]
+
+    def cub(p, x):
+        return (p[0] + x * (p[1] + x * (p[2] + x * p[3]))) % P
+
+    if sum_g != pow(2, m - 1, P) * sum(cub(q, 0) + cub(q, 1) for q in gi) % P:
+        bad.append("sum_g")
+    if beval != sum(cub(q, x) for q, x in zip(gi, alpha)) % P:
+        bad.append("blinding_eval")
+    S = rho * sum_g % P
+    co = [ints(x) for x in out["coeffs"]]
+    for k in range(m):
+        if (cub(co[k], 0) + cub(co[k], 1)) % P != S:
+            bad.append(f"round {k}")
+        S = cub(co[k], alpha[k])
+    if unmont(out["final"]) != S:
+        bad.append("final claim")
+    f_alpha = (S - rho * beval) % P
+    if f_alpha != eq_py(r, alpha) * (mle_eval(a, alpha) * mle_eval(b, alpha) - mle_eval(c, alpha)) % P:
+        bad.append("final equation")
+    return bad
+
+
+def zk_case(m, seed, satisfied=True):
+    rng = random.Random(seed)
     n = 1 << m
-    rng = random.Random(11)
     a = [rng.randrange(P) for _ in range(n)]
     b = [rng.randrange(P) for _ in range(n)]
-    c = [x * y % P for x, y in zip(a, b)]  # satisfied: c = a o b (calculate_witness_bounds)
+    c = [x * y % P for x, y in zip(a, b)]  # c = a o b (calculate_witness_bounds)
+    if not satisfied:
+        c[rng.randrange(n)] ^= 1
     g = [rng.randrange(P) for _ in range(4 * m)]
+    return a, b, c, g
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 7])
+def test_zk_mode_passes_reference_verifier(oracle, m):
+    a, b, c, g = zk_case(m, 11 + m)
     out = oracle.sumcheck_prove(frs(a), frs(b), frs(c), m, frs(g), seed=3)
     assert out["ok"]
+    assert zk_verify(out, m, a, b, c, g) == []
+    # rho is squeezed after sum_g: init, squeeze(m) -> r, absorb(sum_g), squeeze(1)
+    assert np.array_equal(oracle.transcript_run(3, None, m), out["r"])
+
+
+def test_zk_mode_unsatisfied_witness_fails_final_equation(oracle):
+    """With c != a o b the claim rho * sum_g is false and no honest round
+    polynomials reach the verifier's final equation."""
+    m = 5
+    a, b, c, g = zk_case(m, 4, satisfied=False)
+    out = oracle.sumcheck_prove(frs(a), frs(b), frs(c), m, frs(g), seed=9)
+    assert zk_verify(out, m, a, b, c, g) == ["final equation"]
+
+
+@pytest.mark.parametrize("fault,caught", [(1, {"sum_g", "final equation"}),
+                                          (2, {"sum_g", "final equation"}),
+                                          (3, {"final equation"}),
+                                          (4, {"blinding_eval", "final equation"})])
+def test_zk_mode_planted_mistakes_are_caught(oracle, fault, caught):
+    """A wrong prefix multiplier, suffix multiplier, sign of the rho * g mix
+    or a dropped term of blinding_eval, planted in the oracle, fails the
+    verifier's checks (the round identity alone holds by construction)."""
+    m = 5
+    a, b, c, g = zk_case(m, 21)
+    oracle.plant_fault(fault)
+    try:
+        out = oracle.sumcheck_prove(frs(a), frs(b), frs(c), m, frs(g), seed=3)
+    finally:
+        oracle.plant_fault(0)
+    assert out["ok"]  # the identity c0 + c0 + c1 + c2 + c3 = S cannot see these
+    assert set(zk_verify(out, m, a, b, c, g)) == caught
+
+
+def test_zk_mode_needs_a_variable(oracle):
+    z = frs([1])
+    with pytest.raises(ValueError):
+        oracle.sumcheck_prove(z, z, z, 0, frs([1, 2, 3, 4]), seed=1)
+
+
+def test_plain_mode_zero_variables(oracle):
+    """m = 0: the one-element MLEs are padded with a zero (pad_to_pow2_len_min2),
+    so the claim is eq(x=0)(a0 b0 - c0) with the empty eq point's table [1, 0]."""
+    a, b, c = 5, 7, 11
+    out = oracle.sumcheck_prove(frs([a]), frs([b]), frs([c]), 0, None, seed=2)
+    assert out["ok"]
+    assert unmont(out["final"]) == (a * b - c) % P
 
 
 # ---------------------------------------------------------------- GPU
@@ -322,6 +411,17 @@ def test_gpu_eq(gpu, oracle, m):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("m,entries", [(13, 1), (13, 5), (14, 129), (20, 1000), (40, 3), (62, 70)])
+def test_gpu_eq_truncated_large_m(gpu, oracle, m, entries):
+    """A few entries of an eq table over many variables: the half tables hold
+    only the entries the truncated table reaches (at m = 62 the full upper
+    half alone would be 64 GiB)."""
+    r = random_fr(m, seed=m, stream="eqt")
+    print(f"[case] eq m={m} entries={entries}", flush=True)
+    assert np.array_equal(gpu.eq_evaluations(r, entries), oracle.eq_evals(r, entries))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("m", [1, 2, 6, 16])
 def test_gpu_round(gpu, oracle, m):
     n = 1 << m
@@ -362,6 +462,27 @@ def test_gpu_sumcheck_prove(gpu, oracle, m, mode):
     assert np.array_equal(got["coeffs"], exp["coeffs"])
     assert np.array_equal(got["alphas"], exp["alphas"])
     assert np.array_equal(got["final"], exp["final"])
+    assert np.array_equal(got["rho"], exp["rho"])
+    assert np.array_equal(got["blinding_eval"], exp["blinding_eval"])
+    if mode == "ref" and m <= 5:
+        # the device's own output passes the reference verifier
+        got["sum_g"] = exp["sum_g"]
+        assert zk_verify(got, m, ints(a), ints(b), ints(c), ints(g)) == []
+
+
+@pytest.mark.gpu
+def test_gpu_sumcheck_zero_variables(gpu, oracle):
+    """m = 0 in plain mode: one-element MLEs run zero-padded to two (the
+    reference's pad_to_pow2_len_min2), nothing read past the caller's
+    element; a blinding polynomial is rejected (the reference underflows)."""
+    a, b, c = (random_fr(1, seed=k, stream="m0") for k in range(3))
+    print("[case] sumcheck m=0", flush=True)
+    got = gpu.sumcheck_prove(a, b, c, 0, seed=5)
+    exp = oracle.sumcheck_prove(a, b, c, 0, None, seed=5)
+    assert exp["ok"]
+    assert np.array_equal(got["final"], exp["final"])
+    with pytest.raises(gpu.ProvekitGpuError):
+        gpu.sumcheck_prove(a, b, c, 0, blinding=random_fr(4, seed=1, stream="g"), seed=5)
 
 
 @pytest.mark.gpu
