@@ -197,6 +197,19 @@ int pkgpu_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_start
 int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
                             size_t nnz, const uint32_t* cols, const uint64_t* vals,
                             const uint64_t* z, uint64_t* out, void* stream);
+/* No counterpart in the reference: pkgpu_sparse_mul_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results; its
+ * scratch (chunk flags and partial sums) is a fixed device array inside the
+ * library, one per GPU, held until the process ends, so it calls no
+ * hipMalloc/hipFree.  nnz <= PKGPU_SPMV_NOALLOC_MAX_NNZ (else PKGPU_ERR_SIZE).
+ * stream NULL means the device's null stream.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other streams, other threads)
+ * race; serialise them. */
+#define PKGPU_SPMV_NOALLOC_MAX_NNZ ((size_t)1 << 26)
+int pkgpu_sparse_mul_device_noalloc(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                    size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                                    const uint64_t* z, uint64_t* out, void* stream);
 /* SparseMatrix::transpose (provekit/common/src/sparse_matrix.rs:409-449):
  * the num_cols x num_rows matrix whose entries are the (col, row, value)
  * triples in (col, row) order.  t_row_starts receives num_cols entries (row c

@@ -81,6 +81,7 @@ SIGNATURES = {
     "pkgpu_graph_destroy": [_vp],
     "pkgpu_sparse_mul": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp],
     "pkgpu_sparse_mul_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sparse_mul_device_noalloc": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_interner_decode_device": [_vp, _sz, _vp, ctypes.POINTER(_sz), _vp],
     "pkgpu_sparse_matrix_decode_device": [_vp, _sz, _vp, _sz, _vp, _vp],
     "pkgpu_r1cs_decode_device": [_vp, _sz, ctypes.POINTER(_sz), _vp, ctypes.POINTER(_sz), _vp, _vp],

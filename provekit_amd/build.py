@@ -28,6 +28,7 @@ SOURCES = [
     "commit.hip",
     "poseidon2.hip",
     "sumcheck.hip",
+    "noalloc.hip",
     "transpose.hip",
     "wire.hip",
     "batch.hip",

@@ -279,6 +279,7 @@ int pkgpu_init(int device) {
     PK_HIP(hipSetDevice(device));
     pk::device_get(device);
     (void)pk::thread_stream();
+    pk::noalloc_setup(device);
   });
 }
 
@@ -690,6 +691,20 @@ int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* ro
     pk::spmv_dev(csr_view(num_rows, num_cols, row_starts, nnz, cols, vals),
                  reinterpret_cast<const fr_t*>(z), reinterpret_cast<fr_t*>(out),
                  stream_or_thread(stream));
+  });
+}
+int pkgpu_sparse_mul_device_noalloc(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                    size_t nnz, const uint32_t* cols, const uint64_t* vals,
+                                    const uint64_t* z, uint64_t* out, void* stream) {
+  return guarded([&] {
+    check_csr(num_rows, row_starts, nnz, cols, vals);
+    REQ(num_rows == 0 || (out && (num_cols == 0 || z)), pk::PK_ERR_ARG, "sparse_mul: null pointer");
+    REQ(nnz <= PKGPU_SPMV_NOALLOC_MAX_NNZ, pk::PK_ERR_SIZE,
+        "sparse_mul_noalloc: more than 2^26 entries (the fixed scratch holds 2^26)");
+    // NULL: the device's null stream (no per-thread stream is created here)
+    pk::spmv_dev_noalloc(csr_view(num_rows, num_cols, row_starts, nnz, cols, vals),
+                         reinterpret_cast<const fr_t*>(z), reinterpret_cast<fr_t*>(out),
+                         static_cast<hipStream_t>(stream));
   });
 }
 int pkgpu_sparse_transpose(size_t num_rows, size_t num_cols, const uint32_t* row_starts,

@@ -743,22 +743,30 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
-void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {
+uint64_t spmv_chunks(uint64_t nnz) { return nnz ? (nnz + kChunk - 1) / kChunk : 1; }
+
+void spmv_launch(const CsrDev& A, const fr_t* z, fr_t* out, uint32_t* flags, fr_t* partials,
+                 hipStream_t s) {
   if (!A.rows) return;
-  const uint64_t n_chunks = A.nnz ? (A.nnz + kChunk - 1) / kChunk : 1;
+  const uint64_t n_chunks = spmv_chunks(A.nnz);
   // flags[n_chunks], spans[1 + n_chunks], head/tail partials[n_chunks]
-  Scratch flags(4 * (2 * n_chunks + 1), s), part(2 * sizeof(fr_t) * n_chunks, s);
-  uint32_t* fl = flags.as<uint32_t>();
-  uint32_t* spans = fl + n_chunks;
-  fr_t* head = part.as<fr_t>();
+  uint32_t* spans = flags + n_chunks;
+  fr_t* head = partials;
   fr_t* tail = head + n_chunks;
-  k_spmv_clear<<<nblk(n_chunks + 1, 256), 256, 0, s>>>(fl, n_chunks + 1);  // flags and the count
+  k_spmv_clear<<<nblk(n_chunks + 1, 256), 256, 0, s>>>(flags, n_chunks + 1);  // flags and the count
   k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals,
-                                            z, out, fl, spans, n_chunks);
+                                            z, out, flags, spans, n_chunks);
   k_spmv_chunks<<<(unsigned)std::min<uint64_t>(n_chunks, 1024), 256, 0, s>>>(
-      A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals, z, out, fl, head, tail, n_chunks);
+      A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals, z, out, flags, head, tail, n_chunks);
   k_spmv_spans<<<64, 256, 0, s>>>(A.rows, A.row_starts, A.nnz, spans, n_chunks, head, tail, out);
   PK_HIP(hipGetLastError());
+}
+
+void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {
+  if (!A.rows) return;
+  const uint64_t n_chunks = spmv_chunks(A.nnz);
+  Scratch flags(4 * (2 * n_chunks + 1), s), part(2 * sizeof(fr_t) * n_chunks, s);
+  spmv_launch(A, z, out, flags.as<uint32_t>(), part.as<fr_t>(), s);
 }
 
 void hadamard_dev(const fr_t* a, const fr_t* b, fr_t* c, uint64_t n, uint64_t n_pad, hipStream_t s) {

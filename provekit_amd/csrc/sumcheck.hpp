@@ -48,6 +48,18 @@ struct CsrDev {
 };
 
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s);
+// The SpMV launches on caller-provided scratch: flags holds
+// 2 * spmv_chunks(nnz) + 1 u32, partials 2 * spmv_chunks(nnz) elements (both
+// fully rewritten by every call; the flags are cleared by a kernel).
+uint64_t spmv_chunks(uint64_t nnz);
+void spmv_launch(const CsrDev& A, const fr_t* z, fr_t* out, uint32_t* flags, fr_t* partials,
+                 hipStream_t s);
+// The same product with its scratch in a file-scope device array of this
+// library (no allocation); nnz <= kNoAllocSpmvMaxNnz.  Calls overlapping on
+// one device share the array and race.
+constexpr uint64_t kNoAllocSpmvMaxNnz = 1ull << 26;
+void noalloc_setup(int device);
+void spmv_dev_noalloc(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s);
 // SparseMatrix::transpose: t_row_starts gets A.ncols entries, t_cols and
 // t_vals A.nnz (entries in (column, row) order).
 void sparse_transpose_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
