@@ -325,6 +325,46 @@ int pkgpu_commit(const uint64_t* coeffs, size_t n, size_t blowup, int coset, siz
 int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int coset,
                         size_t leaf_elems, uint64_t* codeword, void* nodes, void* stream);
 
+/* WhirR1CSScheme::new_from_dimensions (provekit/r1cs-compiler/src/whir_r1cs.rs:55-84):
+ * the scheme's witness variables m = max(log2 w1, log2 w2, 13) (plus one when
+ * w1's zero padding has no room for 4 m0 coefficients) and sumcheck
+ * variables m0 = max(log2 num_constraints, 1), log2 rounded up. */
+int pkgpu_whir_r1cs_scheme_size(size_t num_witnesses, size_t num_constraints, size_t w1_size,
+                                size_t* m, size_t* m0);
+/* WhirR1CSScheme::commit (provekit/prover/src/whir_r1cs.rs:82-140): the
+ * witness (witness_size Fr) zero-padded to 2^num_vars (pad_to_power_of_two,
+ * then resize; num_vars is whir's num_witness_variables, the scheme's m),
+ * the blinding univariates (m0 x 4 Fr, row-major; NULL for the w2
+ * commitment) written at offset = witness_size, then committed as
+ * pkgpu_commit.  polynomial receives the shaped 2^num_vars elements (the
+ * commitment's `polynomial`, whose [offset, offset + 4 m0) slice the
+ * sumcheck's blinding_eval reads).  Fails with the reference's messages when
+ * the witness exceeds 2^num_vars or the coefficients run past the end.
+ * Host form: polynomial, codeword, nodes may be NULL.  Device form: all
+ * device pointers; witness may equal polynomial, blinding must not overlap
+ * it. */
+int pkgpu_whir_r1cs_commit(const uint64_t* witness, size_t witness_size, size_t num_vars,
+                           const uint64_t* blinding, size_t m0, size_t blowup, int coset,
+                           size_t leaf_elems, uint64_t* polynomial, uint64_t* codeword,
+                           uint8_t* nodes, uint8_t root[32]);
+int pkgpu_whir_r1cs_commit_device(const uint64_t* witness, size_t witness_size, size_t num_vars,
+                                  const uint64_t* blinding, size_t m0, size_t blowup, int coset,
+                                  size_t leaf_elems, uint64_t* polynomial, uint64_t* codeword,
+                                  void* nodes, void* stream);
+
+/* ------------------------------------------------------------ bn254_multiplier */
+/* bn254_multiplier::scalar_mul / scalar_sqr
+ * (skyscraper/bn254-multiplier/src/scalar.rs:12-132) element-wise over n
+ * elements: out[i] = a[i] b[i] 2^-256 (a[i]^2 2^-256) mod p as the reference
+ * computes it (the log-jump reduction, then reduce_ct), below 2^256 - 2p and
+ * not canonical, byte for byte the reference's output.  Inputs: 4 LE u64
+ * limbs each, at most 2^256 - 2p (the reference's contract). */
+int pkgpu_scalar_mul_many(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n);
+int pkgpu_scalar_sqr_many(const uint64_t* a, uint64_t* out, size_t n);
+int pkgpu_scalar_mul_many_device(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n,
+                                 void* stream);
+int pkgpu_scalar_sqr_many_device(const uint64_t* a, uint64_t* out, size_t n, void* stream);
+
 /* ------------------------------------------------------------ batched commit */
 /* Many polynomials of uneven lengths committed together (no counterpart
  * call in the reference, which commits one witness per whir call): a plan

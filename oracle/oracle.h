@@ -42,6 +42,7 @@ void or_sky_permute(const uint64_t l[4], const uint64_t r[4], uint64_t ol[4], ui
 void or_sky_compress_spec(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]);
 void or_sky_compress(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]);
 void or_scalar_sqr(const uint64_t a[4], uint64_t out[4]);
+void or_scalar_mul(const uint64_t a[4], const uint64_t b[4], uint64_t out[4]);
 void or_sky_compress_many(const uint8_t* messages, uint8_t* hashes, size_t count);
 int or_sky_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output);
 /* proof of work (skyscraper/core/src/pow.rs) */
@@ -55,6 +56,11 @@ int or_pow_solve(const uint64_t challenge[4], double difficulty, uint64_t limit,
 void or_merkle_tree(const uint8_t* leaf_digests, size_t num_leaves, uint8_t* nodes);
 int or_commit(const ofr* coeffs, size_t n, size_t blowup, int coset, size_t leaf_elems,
               ofr* codeword, uint8_t* nodes);
+/* WhirR1CSScheme sizes and the witness shaping of WhirR1CSScheme::commit */
+void or_whir_r1cs_params(size_t num_witnesses, size_t num_constraints, size_t w1_size, size_t* m,
+                         size_t* m0);
+int or_whir_r1cs_shape(const ofr* witness, size_t witness_size, size_t num_vars, const ofr* g,
+                       size_t m0, ofr* out);
 
 /* ---- poseidon2 (poseidon2/src) ---- */
 void or_p2_permutation(const ofr in[4], ofr out[4]);

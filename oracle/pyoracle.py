@@ -52,6 +52,9 @@ def lib():
             "or_sky_compress_spec": ([_vp, _vp, _vp], None),
             "or_sky_compress": ([_vp, _vp, _vp], None),
             "or_scalar_sqr": ([_vp, _vp], None),
+            "or_scalar_mul": ([_vp, _vp, _vp], None),
+            "or_whir_r1cs_params": ([_sz, _sz, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz)], None),
+            "or_whir_r1cs_shape": ([_vp, _sz, _sz, _vp, _sz, _vp], _i),
             "or_sky_compress_many": ([_vp, _vp, _sz], None),
             "or_sky_hash_many": ([_sz, _vp, _sz, _vp], _i),
             "or_f64_to_u256": ([ctypes.c_double, _vp], None),
@@ -225,6 +228,34 @@ def scalar_sqr(a) -> np.ndarray:
     o = np.zeros(4, np.uint64)
     lib().or_scalar_sqr(_p(aa), _p(o))
     return o
+
+
+def scalar_mul(a, b) -> np.ndarray:
+    aa, bb = _a(a), _a(b)
+    o = np.zeros(4, np.uint64)
+    lib().or_scalar_mul(_p(aa), _p(bb), _p(o))
+    return o
+
+
+def whir_r1cs_params(num_witnesses: int, num_constraints: int, w1_size: int):
+    m, m0 = ctypes.c_size_t(), ctypes.c_size_t()
+    lib().or_whir_r1cs_params(num_witnesses, num_constraints, w1_size, ctypes.byref(m),
+                              ctypes.byref(m0))
+    return m.value, m0.value
+
+
+def whir_r1cs_shape(witness, num_vars: int, blinding=None) -> np.ndarray:
+    """The witness shaping of WhirR1CSScheme::commit -> (2^num_vars, 4)."""
+    w = _a(witness).reshape(-1, 4)
+    g = _a(blinding).reshape(-1, 4) if blinding is not None else None
+    out = np.zeros((1 << num_vars, 4), np.uint64)
+    rc = lib().or_whir_r1cs_shape(_p(w), w.shape[0], num_vars, _p(g),
+                                  g.shape[0] // 4 if g is not None else 0, _p(out))
+    if rc == -1:
+        raise ValueError("R1CS witness length exceeds scheme capacity")
+    if rc == -2:
+        raise IndexError("blinding coefficients run past the padded witness")
+    return out
 
 
 def sky_compress_many(messages: bytes) -> bytes:

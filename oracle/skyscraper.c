@@ -192,20 +192,12 @@ static const uint64_t I3[4] = {0x9bacb016127cbe4eULL, 0x0b2051fa31944124ULL,
                                0xb064eea46091c76cULL, 0x2b062aaa49f80c7dULL};
 static const uint64_t MU0 = 0xc2e1f593efffffffULL; /* -p^-1 mod 2^64 */
 
-/* bn254_multiplier::scalar_sqr (scalar.rs:12-70): schoolbook square, fold the
- * three low words with 2^-192, 2^-128, 2^-64, one Montgomery step, then
- * subtract 2p if bit 255 is set.  Input < 2p; output < 2^256 - 2p. */
-void or_scalar_sqr(const uint64_t a[4], uint64_t out[4]) {
-  uint64_t t[8] = {0};
-  for (int i = 0; i < 4; i++) {
-    u128 c = 0;
-    for (int j = 0; j < 4; j++) {
-      c += (u128)a[i] * a[j] + t[i + j];
-      t[i + j] = (uint64_t)c;
-      c >>= 64;
-    }
-    t[i + 4] = (uint64_t)c;
-  }
+/* The part of bn254_multiplier::scalar_sqr / scalar_mul after the
+ * schoolbook product (scalar.rs:46-69, :103-131): fold the three low words
+ * of t with 2^-192, 2^-128, 2^-64 (the I3, I2, I1 products, 5 words each,
+ * summed mod 2^320 as `addv` does), one Montgomery step on the low word, keep
+ * words 1..4, then reduce_ct (utils.rs:52-56): subtract 2p if bit 255 is set. */
+static void log_jump_reduce(const uint64_t t[8], uint64_t out[4]) {
   uint64_t s[5] = {t[3], t[4], t[5], t[6], t[7]};
   const uint64_t* I[3] = {I3, I2, I1};
   for (int k = 0; k < 3; k++) {
@@ -235,6 +227,36 @@ void or_scalar_sqr(const uint64_t a[4], uint64_t out[4]) {
   uint64_t r[4] = {s[1], s[2], s[3], s[4]};
   if (r[3] >> 63) sub4(r, PM[2], r);
   memcpy(out, r, 32);
+}
+
+static void schoolbook(const uint64_t a[4], const uint64_t b[4], uint64_t t[8]) {
+  memset(t, 0, 64);
+  for (int i = 0; i < 4; i++) {
+    u128 c = 0;
+    for (int j = 0; j < 4; j++) {
+      c += (u128)a[i] * b[j] + t[i + j];
+      t[i + j] = (uint64_t)c;
+      c >>= 64;
+    }
+    t[i + 4] = (uint64_t)c;
+  }
+}
+
+/* bn254_multiplier::scalar_sqr (scalar.rs:12-70): the full product a*a (all
+ * 16 digit products, as the reference forms them), then log_jump_reduce.
+ * Input < 2p; output < 2^256 - 2p, congruent to a^2 2^-256, not canonical. */
+void or_scalar_sqr(const uint64_t a[4], uint64_t out[4]) {
+  uint64_t t[8];
+  schoolbook(a, a, t);
+  log_jump_reduce(t, out);
+}
+
+/* bn254_multiplier::scalar_mul (scalar.rs:73-132): the same on a*b; inputs
+ * up to 2^256 - 2p (the reference's safe_bn254_montgomery_input). */
+void or_scalar_mul(const uint64_t a[4], const uint64_t b[4], uint64_t out[4]) {
+  uint64_t t[8];
+  schoolbook(a, b, t);
+  log_jump_reduce(t, out);
 }
 
 /* reduce_partial (skyscraper/core/src/reduce.rs:35-41) */
@@ -377,5 +399,50 @@ int or_commit(const ofr* coeffs, size_t n, size_t blowup, int coset, size_t leaf
   or_sky_hash_many(32 * leaf_elems, (const uint8_t*)codeword, leaves, ld);
   or_merkle_tree(ld, leaves, nodes);
   free(ld);
+  return 0;
+}
+
+/* ceil(log2(n)), 0 for n <= 1: next_power_of_two (provekit/common/src/utils/mod.rs:71-79) */
+static size_t ceil_log2(size_t n) {
+  size_t k = 0;
+  while (((size_t)1 << k) < n) k++;
+  return k;
+}
+
+/* WhirR1CSScheme::new_from_dimensions (provekit/r1cs-compiler/src/whir_r1cs.rs:55-84):
+ * m = max(log w1, log w2, 13), m0 = max(log constraints, 1), and one more
+ * variable when w1's zero padding has no room for the 4 m0 blinding
+ * coefficients. */
+void or_whir_r1cs_params(size_t num_witnesses, size_t num_constraints, size_t w1_size, size_t* m,
+                         size_t* m0) {
+  size_t w2 = num_witnesses - w1_size;
+  size_t mr = ceil_log2(w1_size);
+  if (ceil_log2(w2) > mr) mr = ceil_log2(w2);
+  if (mr < 13) mr = 13;
+  size_t z = ceil_log2(num_constraints);
+  if (z < 1) z = 1;
+  if (((size_t)1 << mr) - w1_size < 4 * z) mr++;
+  *m = mr;
+  *m0 = z;
+}
+
+/* The witness shaping of WhirR1CSScheme::commit (provekit/prover/src/whir_r1cs.rs:103-126):
+ * pad_to_power_of_two (utils/mod.rs:84-91), zero-extend to 2^num_vars, and
+ * (g != NULL, the w1 commitment) write the m0 x 4 blinding coefficients at
+ * offset = witness_size.  out: 2^num_vars elements.  Returns -1 if the witness
+ * exceeds 2^num_vars ("R1CS witness length exceeds scheme capacity"), -2 if
+ * the coefficients run past the end (the reference's index panic). */
+int or_whir_r1cs_shape(const ofr* witness, size_t witness_size, size_t num_vars, const ofr* g,
+                       size_t m0, ofr* out) {
+  size_t target = (size_t)1 << num_vars;
+  if (witness_size > target) return -1;
+  size_t padded = (size_t)1 << ceil_log2(witness_size);
+  if (padded < target) padded = target;
+  if (g && witness_size + 4 * m0 > padded) return -2;
+  memset(out, 0, sizeof(ofr) * padded);
+  memcpy(out, witness, sizeof(ofr) * witness_size);
+  if (g)
+    for (size_t i = 0; i < m0; i++)
+      for (size_t j = 0; j < 4; j++) out[witness_size + 4 * i + j] = g[4 * i + j];
   return 0;
 }

@@ -113,6 +113,13 @@ SIGNATURES = {
     "pkgpu_server_destroy": [_vp],
     "pkgpu_commit": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
     "pkgpu_commit_device": [_vp, _sz, _sz, _i, _sz, _vp, _vp, _vp],
+    "pkgpu_whir_r1cs_scheme_size": [_sz, _sz, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "pkgpu_whir_r1cs_commit": [_vp, _sz, _sz, _vp, _sz, _sz, _i, _sz, _vp, _vp, _vp, _vp],
+    "pkgpu_whir_r1cs_commit_device": [_vp, _sz, _sz, _vp, _sz, _sz, _i, _sz, _vp, _vp, _vp, _vp],
+    "pkgpu_scalar_mul_many": [_vp, _vp, _vp, _sz],
+    "pkgpu_scalar_sqr_many": [_vp, _vp, _sz],
+    "pkgpu_scalar_mul_many_device": [_vp, _vp, _vp, _sz, _vp],
+    "pkgpu_scalar_sqr_many_device": [_vp, _vp, _sz, _vp],
 }
 
 
@@ -640,6 +647,55 @@ def commit(coeffs: np.ndarray, blowup: int, coset: bool = True, leaf_elems: int 
           _ptr(cw) if cw is not None else None, _ptr(nodes) if nodes is not None else None,
           _ptr(root))
     return root.tobytes(), cw, nodes
+
+
+def whir_r1cs_scheme_size(num_witnesses: int, num_constraints: int, w1_size: int):
+    """WhirR1CSScheme::new_from_dimensions sizes -> (m, m0)."""
+    m, m0 = ctypes.c_size_t(), ctypes.c_size_t()
+    _call("pkgpu_whir_r1cs_scheme_size", num_witnesses, num_constraints, w1_size,
+          ctypes.byref(m), ctypes.byref(m0))
+    return m.value, m0.value
+
+
+def whir_r1cs_commit(witness, num_vars: int, blinding=None, blowup: int = 4, coset: bool = True,
+                     leaf_elems: int = 16):
+    """WhirR1CSScheme::commit (pkgpu_whir_r1cs_commit).  Returns (root bytes,
+    shaped polynomial (2^num_vars, 4), codeword, nodes (2*leaves-1, 32))."""
+    _ensure_init()
+    w = _fr_array(witness) if len(witness) else np.zeros((0, 4), np.uint64)
+    g = np.ascontiguousarray(blinding, dtype=np.uint64).reshape(-1, 4) if blinding is not None else None
+    m0 = g.shape[0] // 4 if g is not None else 0
+    n = 1 << num_vars
+    N = n * blowup
+    poly = np.zeros((n, 4), np.uint64)
+    cw = np.zeros((N, 4), np.uint64)
+    nodes = np.zeros((2 * (N // leaf_elems) - 1, 32), np.uint8)
+    root = np.zeros(32, np.uint8)
+    _call("pkgpu_whir_r1cs_commit", _ptr(w) if w.size else None, w.shape[0], num_vars,
+          _ptr(g) if g is not None else None, m0, blowup, int(bool(coset)), leaf_elems,
+          _ptr(poly), _ptr(cw), _ptr(nodes), _ptr(root))
+    return root.tobytes(), poly, cw, nodes
+
+
+def scalar_mul_many(a, b) -> np.ndarray:
+    """bn254_multiplier::scalar_mul element-wise (pkgpu_scalar_mul_many)."""
+    _ensure_init()
+    A = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+    B = np.ascontiguousarray(b, dtype=np.uint64).reshape(-1, 4)
+    if A.shape != B.shape:
+        raise ValueError("operands differ in length")
+    out = np.zeros_like(A)
+    _call("pkgpu_scalar_mul_many", _ptr(A), _ptr(B), _ptr(out), A.shape[0])
+    return out
+
+
+def scalar_sqr_many(a) -> np.ndarray:
+    """bn254_multiplier::scalar_sqr element-wise (pkgpu_scalar_sqr_many)."""
+    _ensure_init()
+    A = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros_like(A)
+    _call("pkgpu_scalar_sqr_many", _ptr(A), _ptr(out), A.shape[0])
+    return out
 
 
 # ----------------------------------------------------------------- device memory

@@ -26,6 +26,7 @@ SOURCES = [
     "skyscraper.hip",
     "pow.hip",
     "commit.hip",
+    "scalar.hip",
     "poseidon2.hip",
     "sumcheck.hip",
     "noalloc.hip",

@@ -1,6 +1,8 @@
 // C ABI (include/provekit_gpu.h) and the C++ host mirror (provekit.hpp).
 #include <string.h>
 
+#include <algorithm>
+
 #include <string>
 
 #include "../../include/provekit_gpu.h"
@@ -956,6 +958,113 @@ int pkgpu_commit_device(const uint64_t* coeffs, size_t n, size_t blowup, int cos
     pk::commit_dev(pk::device_current(), reinterpret_cast<const fr_t*>(coeffs), n, blowup, coset,
                    leaf_elems, reinterpret_cast<fr_t*>(codeword), static_cast<fr_t*>(nodes),
                    stream_or_thread(stream));
+  });
+}
+
+int pkgpu_whir_r1cs_scheme_size(size_t num_witnesses, size_t num_constraints, size_t w1_size,
+                                size_t* m, size_t* m0) {
+  return guarded([&] {
+    REQ(m && m0, pk::PK_ERR_ARG, "scheme size: null pointer");
+    REQ(w1_size <= num_witnesses, pk::PK_ERR_SIZE, "w1_size exceeds total witnesses");
+    auto clog = [](size_t n) {
+      size_t k = 0;
+      while ((size_t(1) << k) < n) k++;
+      return k;
+    };
+    size_t mr = std::max<size_t>({clog(w1_size), clog(num_witnesses - w1_size), 13});
+    const size_t z = std::max<size_t>(clog(num_constraints), 1);
+    if ((size_t(1) << mr) - w1_size < 4 * z) mr++;
+    *m = mr;
+    *m0 = z;
+  });
+}
+
+static void whir_commit_args(const uint64_t* witness, size_t witness_size, size_t num_vars,
+                             const uint64_t* blinding, size_t m0) {
+  REQ(witness || witness_size == 0, pk::PK_ERR_ARG, "commit: null witness");
+  REQ(!blinding || m0 > 0, pk::PK_ERR_ARG, "commit: blinding without univariates");
+  REQ(num_vars <= 28, pk::PK_ERR_SIZE, "commit: more than 2^28 coefficients");
+  REQ(witness_size <= (size_t(1) << num_vars), pk::PK_ERR_SIZE,
+      "R1CS witness length exceeds scheme capacity");
+  REQ(!blinding || witness_size + 4 * m0 <= (size_t(1) << num_vars), pk::PK_ERR_SIZE,
+      "index out of bounds: the blinding coefficients run past the padded witness");
+}
+
+int pkgpu_whir_r1cs_commit(const uint64_t* witness, size_t witness_size, size_t num_vars,
+                           const uint64_t* blinding, size_t m0, size_t blowup, int coset,
+                           size_t leaf_elems, uint64_t* polynomial, uint64_t* codeword,
+                           uint8_t* nodes, uint8_t root[32]) {
+  return guarded([&] {
+    REQ(root, pk::PK_ERR_ARG, "commit: null pointer");
+    whir_commit_args(witness, witness_size, num_vars, blinding, m0);
+    const size_t n = size_t(1) << num_vars, N = n * blowup;
+    REQ(pk::is_pow2(blowup) && leaf_elems >= 1 && pk::is_pow2(leaf_elems) && N % leaf_elems == 0,
+        pk::PK_ERR_SIZE, "commit: leaf size must be a power of two dividing the codeword");
+    const size_t leaves = N / leaf_elems, nn = 2 * leaves - 1;
+    const size_t ng = blinding ? 4 * m0 : 0;
+    hipStream_t s = pk::thread_stream();
+    Staged dw(sizeof(fr_t) * std::max<size_t>(witness_size, 1), s), dg(sizeof(fr_t) * (ng + 1), s),
+        dp(sizeof(fr_t) * n, s), dc(sizeof(fr_t) * N, s), dn(sizeof(fr_t) * nn, s);
+    h2d(dw.get<void>(), witness, sizeof(fr_t) * witness_size, s);
+    h2d(dg.get<void>(), blinding, sizeof(fr_t) * ng, s);
+    pk::whir_r1cs_commit_dev(pk::device_current(), dw.get<fr_t>(), witness_size, (int)num_vars,
+                             blinding ? dg.get<fr_t>() : nullptr, m0, blowup, coset,
+                             leaf_elems, dp.get<fr_t>(), dc.get<fr_t>(), dn.get<fr_t>(), s);
+    d2h(root, dn.get<fr_t>() + (nn - 1), 32, s);
+    if (polynomial) d2h(polynomial, dp.get<void>(), sizeof(fr_t) * n, s);
+    if (codeword) d2h(codeword, dc.get<void>(), sizeof(fr_t) * N, s);
+    if (nodes) d2h(nodes, dn.get<void>(), 32 * nn, s);
+    sync(s);
+  });
+}
+int pkgpu_whir_r1cs_commit_device(const uint64_t* witness, size_t witness_size, size_t num_vars,
+                                  const uint64_t* blinding, size_t m0, size_t blowup, int coset,
+                                  size_t leaf_elems, uint64_t* polynomial, uint64_t* codeword,
+                                  void* nodes, void* stream) {
+  return guarded([&] {
+    REQ(polynomial && codeword && nodes, pk::PK_ERR_ARG, "commit: null pointer");
+    whir_commit_args(witness, witness_size, num_vars, blinding, m0);
+    pk::whir_r1cs_commit_dev(pk::device_current(), reinterpret_cast<const fr_t*>(witness),
+                             witness_size, (int)num_vars, reinterpret_cast<const fr_t*>(blinding),
+                             m0, blowup, coset, leaf_elems, reinterpret_cast<fr_t*>(polynomial),
+                             reinterpret_cast<fr_t*>(codeword), static_cast<fr_t*>(nodes),
+                             stream_or_thread(stream));
+  });
+}
+
+static void scalar_host(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n) {
+  REQ(n == 0 || (a && out), pk::PK_ERR_ARG, "scalar: null pointer");
+  if (!n) return;
+  hipStream_t s = pk::thread_stream();
+  Staged da(sizeof(fr_t) * n, s), db(b ? sizeof(fr_t) * n : 0, s), dout(sizeof(fr_t) * n, s);
+  h2d(da.get<void>(), a, sizeof(fr_t) * n, s);
+  if (b) h2d(db.get<void>(), b, sizeof(fr_t) * n, s);
+  pk::scalar_mul_many_dev(da.get<fr_t>(), b ? db.get<fr_t>() : nullptr, dout.get<fr_t>(), n, s);
+  d2h(out, dout.get<void>(), sizeof(fr_t) * n, s);
+  sync(s);
+}
+int pkgpu_scalar_mul_many(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n) {
+  return guarded([&] {
+    REQ(n == 0 || b, pk::PK_ERR_ARG, "scalar: null pointer");
+    scalar_host(a, b, out, n);
+  });
+}
+int pkgpu_scalar_sqr_many(const uint64_t* a, uint64_t* out, size_t n) {
+  return guarded([&] { scalar_host(a, nullptr, out, n); });
+}
+int pkgpu_scalar_mul_many_device(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n,
+                                 void* stream) {
+  return guarded([&] {
+    REQ(n == 0 || (a && b && out), pk::PK_ERR_ARG, "scalar: null pointer");
+    pk::scalar_mul_many_dev(reinterpret_cast<const fr_t*>(a), reinterpret_cast<const fr_t*>(b),
+                            reinterpret_cast<fr_t*>(out), n, stream_or_thread(stream));
+  });
+}
+int pkgpu_scalar_sqr_many_device(const uint64_t* a, uint64_t* out, size_t n, void* stream) {
+  return guarded([&] {
+    REQ(n == 0 || (a && out), pk::PK_ERR_ARG, "scalar: null pointer");
+    pk::scalar_mul_many_dev(reinterpret_cast<const fr_t*>(a), nullptr, reinterpret_cast<fr_t*>(out),
+                            n, stream_or_thread(stream));
   });
 }
 

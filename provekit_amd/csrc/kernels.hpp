@@ -109,5 +109,16 @@ hipGraphExec_t p2_merkle_graph(Device& d, const fr_t* leaves, uint64_t n, fr_t* 
 // left fold, binary Skyscraper tree.  codeword: n*blowup; nodes: 2*leaves-1.
 void commit_dev(Device& d, const fr_t* coeffs, uint64_t n, uint64_t blowup, int coset,
                 uint64_t leaf_elems, fr_t* codeword, fr_t* nodes, hipStream_t s);
+// WhirR1CSScheme::commit: polynomial (2^num_vars) = witness (witness_size),
+// then g (m0 x 4, or none when g is null), then zeros; committed as above.
+// witness may equal polynomial; g must not overlap it.
+void whir_r1cs_shape_dev(const fr_t* witness, uint64_t witness_size, int num_vars, const fr_t* g,
+                         uint64_t m0, fr_t* polynomial, hipStream_t s);
+void whir_r1cs_commit_dev(Device& d, const fr_t* witness, uint64_t witness_size, int num_vars,
+                          const fr_t* g, uint64_t m0, uint64_t blowup, int coset,
+                          uint64_t leaf_elems, fr_t* polynomial, fr_t* codeword, fr_t* nodes,
+                          hipStream_t s);
+// bn254_multiplier::{scalar_mul, scalar_sqr} over n elements (b null: squares).
+void scalar_mul_many_dev(const fr_t* a, const fr_t* b, fr_t* out, uint64_t n, hipStream_t s);
 
 }  // namespace pk

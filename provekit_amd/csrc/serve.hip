@@ -4,7 +4,8 @@
 // slot is free and the job's HBM footprint fits the remaining budget.  An
 // admitted job runs on its own stream: Az, Bz, Cz (SpMV), the whole sumcheck
 // over eq(tau, x)(Az Bz - Cz) with the on-device transcript, and the commit of
-// the witness (RS-NTT + leaves + Skyscraper tree).  Jobs on different streams
+// the witness as WhirR1CSScheme::commit shapes it (zero-padded to at least
+// 2^13 coefficients), RS-NTT + leaves + Skyscraper tree.  Jobs on different streams
 // overlap on the device.  Completion is signalled by a host function enqueued
 // behind the job's work (no HIP call inside it); the scheduler then frees the
 // job's transient buffers, records its latency and admits more work.  The
@@ -26,6 +27,13 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 
 uint64_t align_up(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// The witness commitment has 2^max(m, 13) coefficients: WhirR1CSScheme pads
+// the witness to at least MIN_WHIR_NUM_VARIABLES = 13 variables
+// (provekit/r1cs-compiler/src/whir_r1cs.rs:7,69-73).
+constexpr int kMinWhirVars = 13;
+int commit_vars(const JobSpec& j) { return std::max(j.m, kMinWhirVars); }
+uint64_t commit_len(const JobSpec& j) { return 1ull << commit_vars(j); }
 
 }  // namespace
 
@@ -99,14 +107,17 @@ uint64_t Server::footprint(const JobSpec& job) const {
   t += align_up(32 * n);                  // z
   t += 4 * align_up(32 * n);              // a, b, c, eq
   t += align_up(sizeof(ScState)) + align_up(32 * (sc_partials_elems() + 3));
-  t += align_up(32 * n);                  // coset chunk of the commit
+  const uint64_t cn = commit_len(job);
+  if (cn > n) t += align_up(32 * cn);     // the zero-padded witness polynomial
+  t += align_up(32 * cn);                 // coset chunk of the commit
   return t;
 }
 
 uint64_t Server::submit(const JobSpec& spec) {
   PK_REQUIRE(spec.m >= 1 && spec.m <= kMaxSumcheckVars, PK_ERR_SIZE, "server: m must be 1..28");
   PK_REQUIRE(is_pow2(spec.blowup) && is_pow2(spec.leaf_elems) && spec.leaf_elems >= 2 &&
-                 (1ull << spec.m) * spec.blowup >= spec.leaf_elems,
+                 commit_len(spec) * spec.blowup >= spec.leaf_elems &&
+                 commit_len(spec) * spec.blowup <= (1ull << 28),
              PK_ERR_SIZE, "server: bad commit shape");
   for (int k = 0; k < 3; k++)
     PK_REQUIRE(spec.rs[k] && (spec.nnz[k] == 0 || (spec.cols[k] && spec.vals[k])), PK_ERR_ARG,
@@ -116,7 +127,7 @@ uint64_t Server::submit(const JobSpec& spec) {
   j->spec = spec;
   j->srv = this;
   j->transient_bytes = footprint(spec);
-  const uint64_t N = (1ull << spec.m) * spec.blowup;
+  const uint64_t N = commit_len(spec) * spec.blowup;
   const uint64_t leaves = N / spec.leaf_elems;
   j->resident_bytes = align_up(32 * N) + align_up(32 * (2 * leaves - 1));
   PK_REQUIRE(j->transient_bytes + j->resident_bytes <= budget_, PK_ERR_BUDGET,
@@ -149,7 +160,8 @@ void Server::launch(Job& j) {
   hipStream_t s = streams_[j.stream];
   const JobSpec& sp = j.spec;
   const uint64_t n = 1ull << sp.m;
-  const uint64_t N = n * sp.blowup;
+  const uint64_t cn = commit_len(sp);
+  const uint64_t N = cn * sp.blowup;
   const uint64_t leaves = N / sp.leaf_elems;
   ScState* hst = host_state_[j.stream];
   fr_t* hroot = host_root_[j.stream];
@@ -181,11 +193,15 @@ void Server::launch(Job& j) {
     for (int k = 0; k < 4; k++) abc[k] = (fr_t*)take(32 * n);
     ScState* st = (ScState*)take(sizeof(ScState));
     fr_t* work = (fr_t*)take(32 * (sc_partials_elems() + 3));
+    fr_t* poly = cn > n ? (fr_t*)take(32 * cn) : z;  // shaped in place when no padding
     for (int k = 0; k < 3; k++) spmv_dev(M[k], z, abc[k], s);
     sc_state_init_host(*hst, sp.m, 0, nullptr);
     PK_HIP(hipMemcpyAsync(st, hst, sizeof(ScState), hipMemcpyHostToDevice, s));
     sc_prove_dev(st, abc[0], abc[1], abc[2], abc[3], sp.m, 0, sp.seed, work, s);
-    commit_dev(device_get(device_), z, n, sp.blowup, sp.coset, sp.leaf_elems, j.codeword, j.nodes, s);
+    // WhirR1CSScheme::commit of the witness (plain mode: no blinding
+    // coefficients), after the SpMVs have read z
+    whir_r1cs_commit_dev(device_get(device_), z, n, commit_vars(sp), nullptr, 0, sp.blowup,
+                         sp.coset, sp.leaf_elems, poly, j.codeword, j.nodes, s);
     PK_HIP(hipMemcpyAsync(hst, st, sizeof(ScState), hipMemcpyDeviceToHost, s));
     PK_HIP(hipMemcpyAsync(hroot, j.nodes + (2 * leaves - 2), 32, hipMemcpyDeviceToHost, s));
     PK_HIP(hipFreeAsync(j.transient, s));
@@ -310,7 +326,7 @@ void Server::open(uint64_t id, const uint64_t* leaves_idx, size_t n, fr_t* value
                "server: job not finished, failed or released");
     codeword = j->codeword;
     nodes = j->nodes;
-    N = (1ull << j->spec.m) * j->spec.blowup;
+    N = commit_len(j->spec) * j->spec.blowup;
     leaf_elems = j->spec.leaf_elems;
     j->readers++;  // release() waits until the buffers are no longer read
   }

@@ -43,6 +43,60 @@ def test_scalar_sqr_semantics(oracle):
         assert s % P == a * a * RINV % P
 
 
+OUTPUT_MAX = (1 << 256) - 2 * P  # skyscraper/bn254-multiplier/src/constants.rs:94-99
+I_JUMP = [pow(2, -64 * k, P) for k in (3, 2, 1)]  # the I3, I2, I1 of constants.rs:71-89
+MU0 = (-pow(P, -1, 1 << 64)) % (1 << 64)
+
+
+def scalar_model(a: int, b: int) -> int:
+    """The integer steps of scalar_mul / scalar_sqr (scalar.rs:12-132) written
+    on whole integers rather than 64-bit words: the low three words of a*b are
+    folded with 2^-192, 2^-128, 2^-64 mod p into the high five words (mod
+    2^320, as addv wraps), one Montgomery step cancels the low word, and
+    reduce_ct subtracts 2p when bit 255 is set."""
+    T = a * b
+    M = (1 << 64) - 1
+    s = (T >> 192) + (T & M) * I_JUMP[0] + ((T >> 64) & M) * I_JUMP[1] + ((T >> 128) & M) * I_JUMP[2]
+    s %= 1 << 320
+    m = (MU0 * (s & M)) & M
+    r = ((s + m * P) % (1 << 320)) >> 64
+    if r >> 255:
+        r = (r - 2 * P) % (1 << 256)
+    return r
+
+
+def test_scalar_mul_semantics(oracle):
+    """bn254_multiplier::scalar_mul contract: test_mul_field (inputs up to
+    2^256 - 2p, congruent to l r 2^-256) and test_scalar_mul (Montgomery
+    inputs below p, output below 2^256 - 2p) of scalar.rs:148-186, and the
+    exact bytes against the whole-integer statement of its steps."""
+    rng = random.Random(3)
+    edge = [0, 1, P - 1, P, 2 * P - 1, OUTPUT_MAX - 1, (1 << 255) - 1, 1 << 192]
+    for k in range(600):
+        if k < len(edge) ** 2:
+            a, b = edge[k % len(edge)], edge[k // len(edge)]
+        else:
+            a, b = rng.randrange(OUTPUT_MAX), rng.randrange(OUTPUT_MAX)
+        s = to_int(oracle.scalar_mul(limbs(a), limbs(b)))
+        assert s < OUTPUT_MAX
+        assert s % P == a * b * RINV % P
+        assert s == scalar_model(a, b)
+    for _ in range(200):
+        a, b = rng.randrange(P), rng.randrange(P)
+        s = to_int(oracle.scalar_mul(mont(a), mont(b)))
+        assert s % P == a * b * R % P
+
+
+def test_scalar_sqr_is_mul_by_itself(oracle):
+    """scalar_sqr forms the same 16 digit products as scalar_mul(a, a)
+    (scalar.rs:16-43) and the same reduction, so the bytes agree."""
+    rng = random.Random(4)
+    for _ in range(300):
+        a = rng.randrange(2 * P)
+        assert to_int(oracle.scalar_sqr(limbs(a))) == to_int(oracle.scalar_mul(limbs(a), limbs(a)))
+        assert to_int(oracle.scalar_sqr(limbs(a))) == scalar_model(a, a)
+
+
 def test_root_of_unity(oracle):
     # 5^((p-1)/2^28) is the 2^28-th root; get_root_of_unity(2^k) = it^(2^(28-k))
     t = (P - 1) >> 28

@@ -34,11 +34,17 @@ def make_job(m, seed):
     return mats, z
 
 
+def commit_vars(m):
+    """The witness commitment has at least 2^13 coefficients (WhirR1CSScheme's
+    MIN_WHIR_NUM_VARIABLES, provekit/r1cs-compiler/src/whir_r1cs.rs:7)."""
+    return max(m, 13)
+
+
 def expected(oracle, m, mats, z, seed):
     n = 1 << m
     abc = [oracle.sparse_mul(n, n, rs, cs, vs, z) for rs, cs, vs in mats]
     sc = oracle.sumcheck_prove(abc[0], abc[1], abc[2], m, None, seed=seed)
-    root, cw, nodes = oracle.commit(z, 4, True, 16)
+    root, cw, nodes = oracle.commit(oracle.whir_r1cs_shape(z, commit_vars(m)), 4, True, 16)
     return sc, root, nodes
 
 
@@ -56,11 +62,11 @@ def test_job_matches_oracle_and_openings(gpu, oracle):
     assert np.array_equal(res["r"], sc["r"])
     assert np.array_equal(res["final"], sc["final"])
     # openings: leaf values and sibling path from the resident tree
-    leaves = (4 << m) // 16
+    leaves = (4 << commit_vars(m)) // 16
     depth = leaves.bit_length() - 1
     idx = [0, 1, 37, leaves - 1]
     vals, paths = srv.open(jid, idx, 16, depth)
-    _, cw, _ = oracle.commit(z, 4, True, 16)
+    _, cw, _ = oracle.commit(oracle.whir_r1cs_shape(z, commit_vars(m)), 4, True, 16)
     for q, i in enumerate(idx):
         assert np.array_equal(vals[q], cw[16 * i:16 * i + 16])
         off, n_l = 0, leaves
@@ -110,7 +116,7 @@ def test_budget_admission(gpu):
     assert e.value.code == pk.ERR_BUDGET
     tiny.destroy()
     # a budget that holds one job at a time serialises admission
-    srv = gpu.Server(hbm_budget_bytes=4 << 20, slots=4)  # one m=12 job is ~2.6 MB
+    srv = gpu.Server(hbm_budget_bytes=4 << 20, slots=4)  # one m=12 job is ~3.6 MB
     ids = [srv.submit(m, mats, z, seed=s) for s in range(3)]
     outs = []
     for j in ids:
@@ -131,7 +137,7 @@ def test_release_waits_for_concurrent_openings(gpu):
     mats, z = make_job(m, 11)
     jid = srv.submit(m, mats, z, seed=3)
     srv.wait(jid, timeout_ms=60000)
-    leaves = (4 << m) // 16
+    leaves = (4 << commit_vars(m)) // 16
     depth = leaves.bit_length() - 1
     ref_vals, ref_paths = srv.open(jid, [0, 5, leaves - 1], 16, depth)
     errors, opened = [], [0]
