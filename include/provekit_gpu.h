@@ -365,6 +365,15 @@ int pkgpu_scalar_mul_many_device(const uint64_t* a, const uint64_t* b, uint64_t*
                                  void* stream);
 int pkgpu_scalar_sqr_many_device(const uint64_t* a, uint64_t* out, size_t n, void* stream);
 
+/* ------------------------------------------------------------ synthetic inputs */
+/* No counterpart in the reference (its benchmarks draw Fr::rand from
+ * rand_chacha on the host): n seeded uniform field elements on the device.
+ * out[i] = the ChaCha20 block (RFC 8439, nonce 0) with counter first + i
+ * under `key` (32 bytes), read as a 512-bit little-endian integer mod p, used
+ * as the Montgomery limbs.  first + n <= 2^32. */
+int pkgpu_random_fr_device(const uint8_t key[32], uint64_t first, size_t n, uint64_t* out,
+                           void* stream);
+
 /* ------------------------------------------------------------ batched commit */
 /* Many polynomials of uneven lengths committed together (no counterpart
  * call in the reference, which commits one witness per whir call): a plan

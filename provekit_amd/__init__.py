@@ -116,6 +116,7 @@ SIGNATURES = {
     "pkgpu_whir_r1cs_scheme_size": [_sz, _sz, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
     "pkgpu_whir_r1cs_commit": [_vp, _sz, _sz, _vp, _sz, _sz, _i, _sz, _vp, _vp, _vp, _vp],
     "pkgpu_whir_r1cs_commit_device": [_vp, _sz, _sz, _vp, _sz, _sz, _i, _sz, _vp, _vp, _vp, _vp],
+    "pkgpu_random_fr_device": [_vp, ctypes.c_uint64, _sz, _vp, _vp],
     "pkgpu_scalar_mul_many": [_vp, _vp, _vp, _sz],
     "pkgpu_scalar_sqr_many": [_vp, _vp, _sz],
     "pkgpu_scalar_mul_many_device": [_vp, _vp, _vp, _sz, _vp],
@@ -675,6 +676,19 @@ def whir_r1cs_commit(witness, num_vars: int, blinding=None, blowup: int = 4, cos
           _ptr(g) if g is not None else None, m0, blowup, int(bool(coset)), leaf_elems,
           _ptr(poly), _ptr(cw), _ptr(nodes), _ptr(root))
     return root.tobytes(), poly, cw, nodes
+
+
+def random_fr_device(n: int, seed: int, stream_name: str = "", first: int = 0,
+                     out: "DeviceBuffer" = None, stream=None) -> "DeviceBuffer":
+    """n seeded uniform Fr elements generated on the device
+    (pkgpu_random_fr_device; key and definition in inputs.wide_key /
+    inputs.random_fr_wide)."""
+    from .inputs import wide_key
+    _ensure_init()
+    key = np.frombuffer(wide_key(seed, stream_name), dtype=np.uint8).copy()
+    buf = out if out is not None else DeviceBuffer(32 * max(n, 1))
+    _call("pkgpu_random_fr_device", _ptr(key), first, n, buf.ptr, stream)
+    return buf
 
 
 def scalar_mul_many(a, b) -> np.ndarray:

@@ -27,6 +27,7 @@ SOURCES = [
     "pow.hip",
     "commit.hip",
     "scalar.hip",
+    "inputs.hip",
     "poseidon2.hip",
     "sumcheck.hip",
     "noalloc.hip",

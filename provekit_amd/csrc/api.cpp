@@ -1068,6 +1068,14 @@ int pkgpu_scalar_sqr_many_device(const uint64_t* a, uint64_t* out, size_t n, voi
   });
 }
 
+int pkgpu_random_fr_device(const uint8_t key[32], uint64_t first, size_t n, uint64_t* out,
+                           void* stream) {
+  return guarded([&] {
+    REQ(key && (n == 0 || out), pk::PK_ERR_ARG, "random_fr: null pointer");
+    pk::random_fr_dev(key, first, n, reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
+
 int pkgpu_batch_plan_create(size_t count, const uint64_t* const* coeffs, const size_t* n,
                             const size_t* blowup, const int* coset, size_t leaf_elems,
                             uint64_t* const* codewords, void* const* nodes, int tile_log,

@@ -118,6 +118,9 @@ void whir_r1cs_commit_dev(Device& d, const fr_t* witness, uint64_t witness_size,
                           const fr_t* g, uint64_t m0, uint64_t blowup, int coset,
                           uint64_t leaf_elems, fr_t* polynomial, fr_t* codeword, fr_t* nodes,
                           hipStream_t s);
+// Synthetic inputs: element i = ChaCha20 block first + i under `key`, as a
+// 512-bit integer mod p (inputs.hip).
+void random_fr_dev(const uint8_t key[32], uint64_t first, uint64_t n, fr_t* out, hipStream_t s);
 // bn254_multiplier::{scalar_mul, scalar_sqr} over n elements (b null: squares).
 void scalar_mul_many_dev(const fr_t* a, const fr_t* b, fr_t* out, uint64_t n, hipStream_t s);
 

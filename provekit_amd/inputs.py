@@ -77,6 +77,24 @@ def random_fr(n: int, seed: int, stream: str = "") -> np.ndarray:
     return out
 
 
+def wide_key(seed: int, stream: str = "") -> bytes:
+    """The ChaCha20 key of the device generator (pkgpu_random_fr_device)."""
+    return hashlib.sha256(int(seed).to_bytes(8, "little") + stream.encode() + b"/wide").digest()
+
+
+def random_fr_wide(n: int, seed: int, stream: str = "", first: int = 0) -> np.ndarray:
+    """The device generator's elements first .. first + n - 1 on the host:
+    element i is ChaCha20 block first + i (nonce 0) as a 512-bit LE integer
+    mod p, used as the Montgomery limbs.  Python integers: for checks of
+    the device generator, not for bulk data."""
+    ks = chacha20_blocks(wide_key(seed, stream), first, n)
+    out = np.empty((n, 4), dtype=np.uint64)
+    for i in range(n):
+        v = int.from_bytes(ks[64 * i: 64 * i + 64].tobytes(), "little") % P
+        out[i] = [(v >> (64 * j)) & 0xFFFFFFFFFFFFFFFF for j in range(4)]
+    return out
+
+
 def to_int(limbs) -> int:
     a = np.asarray(limbs, dtype=np.uint64).reshape(4)
     return sum(int(a[i]) << (64 * i) for i in range(4))

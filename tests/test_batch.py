@@ -70,3 +70,21 @@ def test_batch_matches_plain_device_commits(gpu):
     for i, (coeffs, b, c) in enumerate(host):
         root, _, _ = gpu.commit(coeffs, b, c, 16, want_codeword=False, want_nodes=False)
         assert root == roots[i], i
+
+
+def test_bench_batch_full_size(gpu, oracle):
+    """The benchmark's own uneven batch at full size (64 polynomials, k in
+    [12, 22], blowup 2 or 4, independent coefficients per polynomial): every
+    root against the oracle's plain commit of that polynomial."""
+    from provekit_amd import workloads
+    seed = 3
+    polys = workloads.batch_polys(gpu, seed)
+    ks, blowups = workloads.batch_shapes(seed)
+    print(f"[case] bench batch ks={ks}", flush=True)
+    batch = gpu.BatchCommit(polys, leaf_elems=16)
+    batch.run()
+    roots = batch.roots()
+    for i, (buf, n, b, c) in enumerate(polys):
+        coeffs = buf.download(shape=(n, 4))
+        eroot, _, _ = oracle.commit(coeffs, b, c, 16)
+        assert roots[i] == eroot, (i, ks[i], b)

@@ -1,6 +1,7 @@
 """The seeded input generator: ChaCha20 block function against RFC 8439 and
 the sampling contract."""
 import numpy as np
+import pytest
 
 from provekit_amd import inputs
 from tests.util import P, to_int
@@ -33,3 +34,25 @@ def test_random_fr_contract():
     assert not np.array_equal(x, z)
     # prefix-stable: a longer draw starts with the shorter one
     assert np.array_equal(inputs.random_fr(1500, seed=7, stream="t")[:1000], x)
+
+
+def test_random_fr_wide_contract():
+    """The device generator's host statement: element i is ChaCha20 block
+    first + i as a 512-bit integer mod p; sub-ranges agree with the whole."""
+    x = inputs.random_fr_wide(200, seed=3, stream="w")
+    assert all(to_int(v) < P for v in x)
+    blk = inputs.chacha20_blocks(inputs.wide_key(3, "w"), 5, 1)
+    assert to_int(x[5]) == int.from_bytes(blk.tobytes(), "little") % P
+    assert np.array_equal(inputs.random_fr_wide(50, seed=3, stream="w", first=100), x[100:150])
+    assert not np.array_equal(inputs.random_fr_wide(200, seed=4, stream="w"), x)
+
+
+@pytest.mark.gpu
+def test_gpu_random_fr_device(gpu):
+    """pkgpu_random_fr_device against its host statement, whole and from an
+    offset (the offset is the block counter)."""
+    n = 3000
+    got = gpu.random_fr_device(n, seed=11, stream_name="dev").download(shape=(n, 4))
+    assert np.array_equal(got, inputs.random_fr_wide(n, seed=11, stream="dev"))
+    got = gpu.random_fr_device(257, seed=11, stream_name="dev", first=12345).download(shape=(257, 4))
+    assert np.array_equal(got, inputs.random_fr_wide(257, seed=11, stream="dev", first=12345))

@@ -138,6 +138,27 @@ def test_gpu_merkle_2p24_sampled_paths(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_gpu_merkle_2p24_full_root(gpu, oracle):
+    """The benchmark's Poseidon2 tree at full size (2^24 uniform leaves from
+    the device generator, replayed from the captured graph): every node,
+    the root included, against the oracle's whole tree."""
+    from provekit_amd import workloads
+    n = 1 << 24
+    d_leaves = workloads.p2_leaves(gpu, seed=2)
+    d_nodes = gpu.DeviceBuffer(32 * (2 * n - 1))
+    print("[case] poseidon2 merkle 2^24 full tree vs oracle", flush=True)
+    g = gpu.poseidon2_merkle_graph(d_leaves, n, d_nodes)
+    g.launch()
+    gpu.synchronize()
+    leaves = d_leaves.download(shape=(n, 4))
+    exp = oracle.p2_merkle_levels(leaves)
+    got = d_nodes.download(shape=(2 * n - 1, 4))
+    assert np.array_equal(got[-1], exp[-1])  # the root
+    assert np.array_equal(got, exp)
+    g.destroy()
+
+
+@pytest.mark.gpu
 def test_gpu_merkle_graph_replays(gpu, oracle):
     # the captured graph, replayed several times with eager work in between,
     # gives the eager result every time

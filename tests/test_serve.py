@@ -164,3 +164,24 @@ def test_release_waits_for_concurrent_openings(gpu):
     assert not errors, errors
     assert opened[0] >= 5
     srv.destroy()
+
+
+@pytest.mark.parametrize("m", [16, 22])
+def test_job_full_size(gpu, oracle, m):
+    """Serving jobs at the workload's sizes (m = 16 and m = 22, uniform values
+    and witness from the device generator): sumcheck transcript and witness
+    root against the oracle."""
+    from provekit_amd import workloads
+    mats, z = workloads.r1cs_job(gpu, m, seed=5)
+    srv = gpu.Server(hbm_budget_bytes=32 << 30, slots=2)
+    print(f"[case] serving job m={m}", flush=True)
+    jid = srv.submit(m, mats, z, seed=77)
+    res = srv.wait(jid, timeout_ms=120000)
+    srv.release(jid)
+    srv.destroy()
+    host, zh = workloads.download_job(mats, z, m)
+    sc, root, _ = expected(oracle, m, host, zh, 77)
+    assert sc["ok"]
+    assert res["root"] == root
+    for key in ("coeffs", "alphas", "r", "final"):
+        assert np.array_equal(res[key], sc[key]), key
