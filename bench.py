@@ -44,6 +44,11 @@ MODMUL_PEAK_G = VALU_PEAK_TOPS * 1e3 / 240
 # multiply-adds and 100 - 16 carry adds, 184 VALU instructions.  The
 # Skyscraper kernels run squarings only, so they are priced against this.
 MODSQR_PEAK_G = VALU_PEAK_TOPS * 1e3 / 184
+# Poseidon2 runs in 9 x 29-bit limbs: a product is 162 multiply-adds into one
+# accumulator plus the quotient digits and limb masks, 218 VALU instructions
+# as compiled for gfx950 (a squaring fewer), so its speed of light is
+# VALU peak / 218.
+MOD29_PEAK_G = VALU_PEAK_TOPS * 1e3 / 218
 
 
 class Hip:
@@ -155,19 +160,11 @@ def run_batched(pk, hip, s, seed, steps, warmup, leaf_elems=16):
     """Uneven batch: 64 polynomials, k ~ U{12..22}, blowup 2 or 4 by coin flip, coset
     NTT, one tree each; batched (one persistent hashing launch) vs 64 plain
     per-call commits on the same GPU, everything resident in HBM."""
-    from provekit_amd.inputs import random_fr
+    from provekit_amd import workloads
 
-    rng = np.random.default_rng(seed)
-    ks = rng.integers(12, 23, size=64)
-    blowups = rng.choice([2, 4], size=64)
-    # coefficients: seeded rotations of one 2^22-element ChaCha20 pool
-    pool = random_fr(1 << 22, seed=seed, stream="batch-pool")
-    offs = rng.integers(0, 1 << 22, size=64)
-    polys = []
-    for k, b, off in zip(ks, blowups, offs):
-        c = np.roll(pool, -int(off), axis=0)[: 1 << int(k)]
-        polys.append((pk.DeviceBuffer.from_array(c), 1 << int(k), int(b), True))
-    del pool
+    # 64 polynomials of independent uniform coefficients (device generator)
+    ks, blowups = workloads.batch_shapes(seed)
+    polys = workloads.batch_polys(pk, seed)
     batch = pk.BatchCommit(polys, leaf_elems=leaf_elems)
     total = int(sum(1 << int(k) for k in ks))
 
@@ -214,7 +211,7 @@ def run_batched(pk, hip, s, seed, steps, warmup, leaf_elems=16):
     best = min(batch_ms, graph_ms)
     return {
         "workload": "64 polys, k~U{12..22}, blowup 2/4 coin flip, coset RS-NTT, 16-element leaves, "
-                    "Skyscraper tree each; coefficients = seeded rotations of a 2^22 ChaCha20 pool",
+                    "Skyscraper tree each; independent uniform coefficients per polynomial",
         "total_coefficients": total,
         "ks": [int(k) for k in ks], "blowups": [int(b) for b in blowups],
         "plain_ms": plain_ms, "plain_elements_per_s": total / (plain_ms / 1e3),
@@ -227,19 +224,6 @@ def run_batched(pk, hip, s, seed, steps, warmup, leaf_elems=16):
     }
 
 
-def pool_fr(n, seed, stream):
-    """n Fr elements: seeded rotations of one ChaCha20 pool of at most 2^22
-    (the generator costs ~1 us per element on the host)."""
-    from provekit_amd.inputs import random_fr
-    m = min(n, 1 << 22)
-    pool = random_fr(m, seed=seed, stream=stream)
-    if n == m:
-        return pool
-    rng = np.random.default_rng(seed)
-    parts = [np.roll(pool, -int(rng.integers(0, m)), axis=0) for _ in range((n + m - 1) // m)]
-    return np.ascontiguousarray(np.concatenate(parts)[:n])
-
-
 def base_line(metric, value, unit, steps, warmup, ms_per_step, higher, dtype, config, extra):
     out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
            "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
@@ -250,23 +234,27 @@ def base_line(metric, value, unit, steps, warmup, ms_per_step, higher, dtype, co
 
 
 def workload_ntt(pk, hip, s, args):
-    """Plain forward NTT of 2^16 uniform Fr (natural -> bit-reversed)."""
+    """Plain forward NTT of 2^16 uniform Fr (natural -> bit-reversed): two
+    LDS-staged pass launches per transform (k_ntt_pass, the tuned shape),
+    timed with HIP events on the launching stream; parity of a fresh
+    transform against the oracle; the oracle's transform timed on the host."""
     from oracle import pyoracle
     from provekit_amd.inputs import random_fr
 
     n = 1 << 16
     v = random_fr(n, seed=args.seed, stream="ntt-2^16")
     d = pk.DeviceBuffer.from_array(v)
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):
         pk.ntt_nr_device(d, n, n, 1, s)
     hip.sync(s)
+    steps = max(args.steps, 20)
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, s)
-    for _ in range(args.steps):
+    for _ in range(steps):
         pk.ntt_nr_device(d, n, n, 1, s)
     hip.record(e1, s)
     hip.sync(s)
-    ms = hip.elapsed_ms(e0, e1) / args.steps
+    ms = hip.elapsed_ms(e0, e1) / steps
     # parity of one fresh transform against the oracle
     d.upload(v)
     pk.ntt_nr_device(d, n, n, 1, s)
@@ -279,28 +267,41 @@ def workload_ntt(pk, hip, s, args):
         w = v.copy()
         pyoracle.ntt_nr(w, n, 1)
         reps += 1
-        if time.perf_counter() - t0 > min(args.cpu_budget, 5.0):
+        if time.perf_counter() - t0 > min(args.cpu_budget, 3.0):
             break
     cpu_ms = (time.perf_counter() - t0) * 1e3 / reps
     pyoracle.ntt_nr(ref, n, 1)
     modmul = (n // 2) * 16
+    achieved = modmul / (ms / 1e3) / 1e9
+    roofline = {
+        "bound": "valu", "kernel": "k_ntt_pass (2 launches per 2^16 transform)",
+        "achieved": achieved, "peak": MODMUL_PEAK_G, "unit": "Gmodmul/s",
+        "frac": achieved / MODMUL_PEAK_G, "traffic": load_traffic("k_ntt_pass:2p16"),
+        "work_per_launch": f"{modmul} butterfly products per transform (n/2 x 16 stages)",
+        "peak_basis": f"VALU int32 issue {VALU_PEAK_TOPS:.1f} T lane-instr/s / 240 instr per 8x32 Montgomery product",
+        "hbm_achieved_GBs": 2 * 32 * n / (ms / 1e3) / 1e9,
+        "hbm_frac": 2 * 32 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+    }
     return base_line("Fr elements/s transformed (forward NTT 2^16)", n / (ms / 1e3), "Fr elements/s",
-                     args.steps, args.warmup, ms, True, "u32",
+                     steps, args.warmup, ms, True, "u32",
                      {"workload": "forward NTT ntt_nr, 2^16 Fr, natural -> bit-reversed"},
-                     {"modmul_rate_G": modmul / (ms / 1e3) / 1e9, "matches_oracle": bool(np.array_equal(got, ref)),
+                     {"modmul_rate_G": achieved, "matches_oracle": bool(np.array_equal(got, ref)),
+                      "roofline": roofline,
                       "cpu_baseline": {"value": n / (cpu_ms / 1e3), "unit": "Fr elements/s",
                                        "cores": pyoracle.threads(), "kind": "port",
-                                       "sample": f"{reps} transforms of 2^16, mean {cpu_ms:.2f} ms"}})
+                                       "sample": f"the full workload: {reps} transforms of 2^16, mean {cpu_ms:.2f} ms"}})
 
 
 def workload_p2merkle(pk, hip, s, args):
-    """Poseidon2 2-to-1 Merkle tree over 2^24 uniform Fr leaves, the
-    level launches replayed from one captured hipGraph."""
+    """Poseidon2 2-to-1 Merkle tree over 2^24 uniform Fr leaves (device
+    generator), the level launches replayed from one captured hipGraph; the
+    oracle builds the whole tree on the host (the CPU baseline) and its root
+    is compared with the device's."""
     from oracle import pyoracle
+    from provekit_amd import workloads
 
     n = 1 << 24
-    leaves = pool_fr(n, args.seed, "p2-leaves")
-    d_leaves = pk.DeviceBuffer.from_array(leaves)
+    d_leaves = workloads.p2_leaves(pk, args.seed)
     d_nodes = pk.DeviceBuffer(32 * (2 * n - 1))
     g = pk.poseidon2_merkle_graph(d_leaves, n, d_nodes)
     for _ in range(max(1, args.warmup)):
@@ -315,65 +316,61 @@ def workload_p2merkle(pk, hip, s, args):
     hip.sync(s)
     ms = hip.elapsed_ms(e0, e1) / steps
     root = d_nodes.download(np.uint64, nbytes=32 * (2 * n - 1))[-4:]
-    # CPU: a 2^16-leaf subtree (bounded sample), and parity of that subtree's root
-    sub = 1 << 16
-    t0 = time.perf_counter()
-    nodes = pyoracle.p2_merkle_levels(leaves[:sub])
-    cpu_s = time.perf_counter() - t0
-    sub_gpu_root, _ = pk.poseidon2_merkle(leaves[:sub], want_nodes=False)
+    g.destroy()
     perms = n - 1
+    achieved = perms * 488 / (ms / 1e3) / 1e9
+    roofline = {
+        "bound": "valu", "kernel": "k_p2_level (+ k_p2_tail), 24 levels from one hipGraph",
+        "achieved": achieved, "peak": MOD29_PEAK_G, "unit": "Gmodmul/s",
+        "frac": achieved / MOD29_PEAK_G, "traffic": load_traffic("k_p2_level"),
+        "work_per_launch": f"{perms} permutations x 488 products (96 full-round + 392 partial-round)",
+        "peak_basis": f"VALU int32 issue {VALU_PEAK_TOPS:.1f} T lane-instr/s / 218 instr per 9x29-bit Montgomery product",
+        "hbm_achieved_GBs": (32 * (2 * n - 1) + 32 * (n - 1)) / (ms / 1e3) / 1e9,
+    }
+    cpu = None
+    matches = None
+    if not getattr(args, "no_cpu", False):
+        leaves = d_leaves.download(shape=(n, 4))
+        t0 = time.perf_counter()
+        nodes = pyoracle.p2_merkle_levels(leaves)
+        cpu_s = time.perf_counter() - t0
+        matches = bool(np.array_equal(np.asarray(root, dtype=np.uint64), nodes[-1]))
+        del nodes, leaves
+        cpu = {"value": perms / cpu_s, "unit": "hashes/s", "cores": pyoracle.threads(), "kind": "port",
+               "sample": f"the full workload: the whole 2^24-leaf tree ({perms} permutations) once, {cpu_s:.1f} s"}
     return base_line("Poseidon2 hashes/s (2-to-1 Merkle over 2^24 Fr leaves, hipGraph)", perms / (ms / 1e3),
                      "hashes/s", steps, args.warmup, ms, True, "u32",
                      {"workload": "Poseidon2 t=4 2-to-1 Merkle tree over 2^24 Fr leaves, one captured hipGraph"},
-                     {"modmul_rate_G": perms * 488 / (ms / 1e3) / 1e9,
+                     {"modmul_rate_G": achieved,
                       "root": np.asarray(root, dtype=np.uint64).tobytes().hex(),
-                      "subtree_2p16_matches_oracle": bool(np.array_equal(sub_gpu_root, nodes[-1])),
-                      "cpu_baseline": {"value": (sub - 1) / cpu_s, "unit": "hashes/s",
-                                       "cores": pyoracle.threads(), "kind": "port",
-                                       "sample": f"2^16-leaf subtree ({sub - 1} permutations) in {cpu_s:.2f} s"}})
-
-
-def csr_template(rng, n, vals_pool):
-    k = rng.integers(1, 5, size=n)
-    cand = np.sort(rng.integers(0, n, size=(n, 4)), axis=1)
-    keep = np.ones((n, 4), bool)
-    keep[:, 1:] = cand[:, 1:] != cand[:, :-1]
-    keep &= np.arange(4)[None, :] < k[:, None]
-    rs = np.zeros(n, np.uint32)
-    rs[1:] = np.cumsum(keep.sum(axis=1))[:-1]
-    cols = cand[keep].astype(np.uint32)
-    off = int(rng.integers(0, vals_pool.shape[0]))
-    reps = (cols.shape[0] + vals_pool.shape[0] - 1) // vals_pool.shape[0] + 1
-    vals = np.ascontiguousarray(np.tile(np.roll(vals_pool, -off, axis=0), (reps, 1))[: cols.shape[0]])
-    return rs, cols, vals
+                      "matches_oracle": matches, "roofline": roofline, "cpu_baseline": cpu})
 
 
 def workload_serving(pk, hip, s, args):
     """Serving: Poisson stream of R1CS jobs, m ~ U{16..22}, 1..4 nonzeros per
-    row; each job: Az, Bz, Cz, the full sumcheck, the witness commit."""
+    row, uniform values and witness (one input set per size, device
+    generator); each job: Az, Bz, Cz, the full sumcheck, the witness commit.
+    The warm-up runs one job of each size; the oracle runs the same seven
+    jobs on the host (the CPU baseline: jobs/s at the workload's size mix)
+    and their transcripts and roots are compared with the device's.  The
+    roofline is the jobs' dominant kernel, leaf hashing of the m = 22
+    witness codeword, timed alone with HIP events."""
+    from provekit_amd import workloads
+
     rng = np.random.default_rng(args.seed)
-    pool = pool_fr(1 << 20, args.seed, "serve-pool")
-    templates = {}
-    for m in range(16, 23):
-        n = 1 << m
-        mats = []
-        for _ in range(3):
-            rs, cols, vals = csr_template(rng, n, pool)
-            mats.append(tuple(pk.DeviceBuffer.from_array(x) for x in (rs, cols, vals)))
-        z = np.ascontiguousarray(np.tile(pool, ((n + pool.shape[0] - 1) // pool.shape[0], 1))[:n])
-        z[0] = np.array([0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E,
-                         0x0E0A77C19A07DF2F], dtype=np.uint64)  # z[0] = 1
-        templates[m] = (mats, pk.DeviceBuffer.from_array(z))
+    sizes = list(range(16, 23))
+    templates = {m: workloads.r1cs_job(pk, m, args.seed) for m in sizes}
     jobs = args.jobs
     ms_ = rng.integers(16, 23, size=jobs)
     rate = args.rate
     gaps = rng.exponential(1.0 / rate, size=jobs)
     srv = pk.Server(hbm_budget_bytes=args.budget_gb << 30, slots=args.slots)
-    # warm-up: one job of each size
-    for m in range(16, 23):
+    # warm-up: one job of each size (also the parity sample)
+    warm = {}
+    for m in sizes:
         mats, z = templates[m]
         j = srv.submit(m, mats, z, seed=1)
-        srv.wait(j, timeout_ms=120000)
+        warm[m] = srv.wait(j, timeout_ms=120000)
         srv.release(j)
     t0 = time.perf_counter()
     outstanding, results = [], []
@@ -400,6 +397,54 @@ def workload_serving(pk, hip, s, args):
     srv.destroy()
     lat = [r["latency_ms"] for r in results]
     elems = int(sum(1 << int(m) for m in ms_))
+
+    # dominant kernel alone: leaf hashing of the m = 22 witness codeword
+    # (2^24 elements, 2^20 leaves of 16)
+    N = 4 << 22
+    leaves = N // 16
+    cw = pk.random_fr_device(N, args.seed, "serve-roofline-cw")
+    dg = pk.DeviceBuffer(32 * leaves)
+    for _ in range(2):
+        pk._call("pkgpu_skyscraper_hash_many_device", 32 * 16, cw.ptr, leaves, dg.ptr, s)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, s)
+    for _ in range(5):
+        pk._call("pkgpu_skyscraper_hash_many_device", 32 * 16, cw.ptr, leaves, dg.ptr, s)
+    hip.record(e1, s)
+    hip.sync(s)
+    leaf_ms = hip.elapsed_ms(e0, e1) / 5
+    del cw, dg
+    sq = leaves * 15 * 14
+    achieved = sq / (leaf_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "valu", "kernel": "k_hash_fold (witness commit leaf hashing, m = 22: 2^20 leaves)",
+        "achieved": achieved, "peak": MODSQR_PEAK_G, "unit": "Gmodsqr/s", "frac": achieved / MODSQR_PEAK_G,
+        "traffic": None, "launch_ms": leaf_ms,
+        "work_per_launch": f"{leaves * 15} compressions x 14 Montgomery squarings",
+    }
+
+    cpu = None
+    matches = None
+    if not getattr(args, "no_cpu", False):
+        from oracle import pyoracle
+        t_m, ok = {}, True
+        for m in sizes:
+            mats, z = templates[m]
+            host, zh = workloads.download_job(mats, z, m)
+            n = 1 << m
+            t0c = time.perf_counter()
+            abc = [pyoracle.sparse_mul(n, n, rs, cs, vs, zh) for rs, cs, vs in host]
+            sc = pyoracle.sumcheck_prove(abc[0], abc[1], abc[2], m, None, seed=1)
+            root, _, _ = pyoracle.commit(pyoracle.whir_r1cs_shape(zh, max(m, 13)), 4, True, 16)
+            t_m[m] = time.perf_counter() - t0c
+            ok &= bool(sc["ok"]) and root == warm[m]["root"]
+            ok &= all(np.array_equal(warm[m][k], sc[k]) for k in ("coeffs", "alphas", "r", "final"))
+        matches = ok
+        mean_s = statistics.mean(t_m.values())  # m is uniform over 16..22
+        cpu = {"value": 1.0 / mean_s, "unit": "jobs/s", "cores": pyoracle.threads(), "kind": "port",
+               "sample": "one job of each size m = 16..22 (SpMV x3 + sumcheck + witness commit), "
+                         f"{sum(t_m.values()):.1f} s in all; jobs/s at the uniform size mix",
+               "seconds_by_m": {str(k): v for k, v in t_m.items()}}
     return base_line("jobs/s (R1CS SpMV + sumcheck + witness commit, Poisson arrivals)", jobs / wall,
                      "jobs/s", jobs, 7, wall * 1e3 / jobs, True, "u32",
                      {"workload": f"serving: {jobs} jobs, m~U{{16..22}}, 1-4 nnz/row, Poisson rate {rate}/s, "
@@ -407,7 +452,8 @@ def workload_serving(pk, hip, s, args):
                       "rate_per_s": rate},
                      {"p50_job_ms": pct(lat, 0.5), "p99_job_ms": pct(lat, 0.99),
                       "mean_queue_ms": statistics.mean(r["queue_ms"] for r in results),
-                      "witness_elements_per_s": elems / wall, "peak_in_flight": st["peak_in_flight"]})
+                      "witness_elements_per_s": elems / wall, "peak_in_flight": st["peak_in_flight"],
+                      "matches_oracle": matches, "roofline": roofline, "cpu_baseline": cpu})
 
 
 def workload_pow(pk, hip, s, args):
@@ -416,7 +462,8 @@ def workload_pow(pk, hip, s, args):
     from oracle import pyoracle
 
     bits = args.pow_bits
-    chal = pool_fr(max(1, args.steps) + 2, args.seed, "pow-challenges")
+    from provekit_amd.inputs import random_fr
+    chal = random_fr(max(1, args.steps) + 2, seed=args.seed, stream="pow-challenges")
     pk.pow.solve(chal[-1], 10.0, stream=s)  # warm-up: kernel load, first windows
     hip.sync(s)
     steps = max(1, min(args.steps, 5))
@@ -549,16 +596,12 @@ def workload_sumcheck(pk, hip, s, args):
     sumcheck over eq(r, x)(a b - c) in plain mode, all device-resident."""
     from oracle import pyoracle
 
+    from provekit_amd import workloads
+
     m = args.log_m
     n = 1 << m
-    rng = np.random.default_rng(args.seed)
-    pool = pool_fr(1 << 20, args.seed, "sumcheck-pool")
-    mats_h = [csr_template(rng, n, pool) for _ in range(3)]
-    mats = [tuple(pk.DeviceBuffer.from_array(x) for x in mh) for mh in mats_h]
-    z = np.ascontiguousarray(np.tile(pool, ((n + pool.shape[0] - 1) // pool.shape[0], 1))[:n])
-    z[0] = np.array([0xAC96341C4FFFFFFB, 0x36FC76959F60CD29, 0x666EA36F7879462E,
-                     0x0E0A77C19A07DF2F], dtype=np.uint64)  # z[0] = 1
-    dz = pk.DeviceBuffer.from_array(z)
+    mats, dz = workloads.r1cs_job(pk, m, args.seed)
+    mats_h, z = workloads.download_job(mats, dz, m)
     abc = [pk.DeviceBuffer(32 * n) for _ in range(3)]
     nnz = [int(mh[1].shape[0]) for mh in mats_h]
 
@@ -673,6 +716,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the config-1 NTT, config-4 Poseidon2 and config-5 serving keys")
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy kernel (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -827,6 +872,16 @@ def main():
         cb, croot = cpu_baseline(coeffs, args.blowup, args.leaf, args.cpu_budget)
         out["cpu_baseline"] = cb
         out["root_matches_cpu"] = croot == root
+    if not args.no_extra:
+        # the other single-GPU configurations, each with its own roofline,
+        # full-workload CPU baseline and oracle parity
+        keep = ("metric", "value", "unit", "ms_per_step", "steps", "config", "matches_oracle",
+                "roofline", "cpu_baseline", "modmul_rate_G", "p50_job_ms", "p99_job_ms",
+                "witness_elements_per_s", "peak_in_flight", "mean_queue_ms", "root")
+        for key, fn in (("config1_ntt_2p16", workload_ntt), ("config4_poseidon2_2p24", workload_p2merkle),
+                        ("config5_serving", workload_serving)):
+            line = fn(pk, hip, s, args)
+            out[key] = {k: line[k] for k in keep if k in line}
     print(json.dumps(out), flush=True)
     if args.copy_probe:
         copy_probe()

@@ -100,13 +100,14 @@ __device__ __forceinline__ void norm29(f29& a) {
   }
 }
 
-// a (normalized, value < 2^262) -> a value below 4p, same residue: subtract
-// q p with q a lower estimate of a / p from the top limb (p >> 232 =
-// 3171406.45...; the float quotient is exact to well under one unit and is
-// lowered by one, so the result stays non-negative).
+// a (normalized, value < 2^262, so a.l[8] < 2^30) -> a value below 4p, same
+// residue: subtract q p with q a lower estimate of a / p from the top limb,
+// in integers: q = floor(a.l[8] floor(2^52 / 3171407) / 2^52) - 1, where
+// p >> 232 = 3171406.45..., so q <= floor(a / p) and a - q p < 2.0001 p
+// (checked at both ends of every top-limb value's range in a CPU model).
 __device__ __forceinline__ void reduce29(f29& a) {
-  const float qf = __uint2float_rz(a.l[8]) * (1.0f / 3171407.0f);
-  uint32_t q = (uint32_t)qf;
+  constexpr uint64_t kRecip = (1ull << 52) / 3171407;  // 0x54a4736b
+  uint32_t q = (uint32_t)(((uint64_t)a.l[8] * kRecip) >> 52);
   q = q ? q - 1 : 0;
   int64_t carry = 0;
 #pragma unroll
