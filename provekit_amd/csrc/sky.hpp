@@ -131,6 +131,106 @@ __device__ __forceinline__ fr_t sky_compress(const fr_t& l, const fr_t& r) {
   return sky_compress_v(l, r);
 }
 
+// ---------------------------------------------------------------- table form
+// The reference's fused reduction (reduce_partial_add_rc, reduce.rs:48-55,
+// with MODULUS_N_MINUS_RC, constants.rs:51-58): a round's sum x = r + F(l)
+// (below 4p, no reduction) becomes x - (k p - rc) with k = the multiple of p
+// read off x's top limb, one subtraction chain instead of two reduced
+// additions.  k is per lane, so the 6 candidates k p - rc (k = 0..5) of each
+// round sit in a 3.4 KB LDS table, filled once per workgroup.
+//   k = floor(x_7 / (p_7 + 1)) = (x_7 * 0xa948e8c5) >> 61 exactly for every
+//   32-bit x_7 (checked exhaustively on the CPU); one full-rate
+//   v_mad_u64_u32 and a shift.  k p <= x_7 2^224 <= x, and
+//   x - k p < p + (k + 1) 2^224, so x - k p + rc < 2p for every round
+//   constant (all lie below p - 2^250): the lazy range [0, 2p) is kept.
+// Half-round h (0..17) uses row h: rc[h] (rows 0 and 17 have rc = 0, the
+// input, S-box and output reductions use row 0).  Per round: an 8-limb add,
+// the quotient (2), the LDS address (1) and an 8-limb subtraction: 19 VALU
+// instructions against 56 for two reduced additions with the constant moved
+// into registers.
+constexpr int kSkyTabRows = 18, kSkyTabMults = 6;
+struct SkyTab {
+  uint4 e[kSkyTabRows * kSkyTabMults * 2];  // entry (h, k): e[2 (6h + k)], e[2 (6h + k) + 1]
+};
+
+// Fill the table; every thread of the workgroup calls it, then a barrier.
+__device__ __forceinline__ void sky_tab_fill(SkyTab& tab) {
+  for (int t = threadIdx.x; t < kSkyTabRows * kSkyTabMults; t += blockDim.x) {
+    const int h = t / kSkyTabMults, k = t % kSkyTabMults;
+    uint32_t v[8];
+    uint64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // k p
+      c += (uint64_t)k * p_limb(i);
+      v[i] = (uint32_t)c;
+      c >>= 32;
+    }
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = subb(v[i], kSkyRC[h][i], br);  // - rc, mod 2^256
+    tab.e[2 * t] = make_uint4(v[0], v[1], v[2], v[3]);
+    tab.e[2 * t + 1] = make_uint4(v[4], v[5], v[6], v[7]);
+  }
+}
+
+// floor(x7 / (p_7 + 1)), exact for all 32-bit x7
+__device__ __forceinline__ uint32_t sky_top_mult(uint32_t x7) {
+  uint64_t t, cy;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(t), "=s"(cy) : "v"(x7), "s"(0xa948e8c5u));
+  (void)cy;
+  return (uint32_t)(t >> 61);
+}
+
+// x (any value below 2^256) -> x - (k p - rc[h]) in [0, 2p) (rc = 0: [0, p + 6 2^224))
+__device__ __forceinline__ fr_t sky_tab_reduce(const fr_t& x, const SkyTab& tab, int h) {
+  const uint32_t k = sky_top_mult(x.l[7]);
+  const uint4 a = tab.e[2 * (kSkyTabMults * h) + 2 * k];
+  const uint4 b = tab.e[2 * (kSkyTabMults * h) + 2 * k + 1];
+  fr_t r;
+  uint32_t br = 0;
+  r.l[0] = subb(x.l[0], a.x, br);
+  r.l[1] = subb(x.l[1], a.y, br);
+  r.l[2] = subb(x.l[2], a.z, br);
+  r.l[3] = subb(x.l[3], a.w, br);
+  r.l[4] = subb(x.l[4], b.x, br);
+  r.l[5] = subb(x.l[5], b.y, br);
+  r.l[6] = subb(x.l[6], b.z, br);
+  r.l[7] = subb(x.l[7], b.w, br);
+  return r;
+}
+
+// bar with the table: the raw S-box output (any 256-bit value) comes back
+// below p + 6 2^224
+__device__ __forceinline__ fr_t sky_bar_t(fr_t x, const SkyTab& tab) {
+  fr_canon(x);
+  fr_t y;
+#pragma unroll
+  for (int i = 0; i < 8; i++) y.l[i] = sbox4(x.l[(i + 4) & 7]);
+  return sky_tab_reduce(y, tab, 0);
+}
+
+// compress(l, r) with the table reductions (generic.rs:77-102 order: r +=
+// F(l), then the fused reduction with rc[h]); canonical output, the same
+// bytes as sky_compress.
+__device__ __forceinline__ fr_t sky_compress_t(fr_t l, fr_t r, const SkyTab& tab) {
+  l = sky_tab_reduce(l, tab, 0);
+  r = sky_tab_reduce(r, tab, 0);
+  const fr_t t = l;
+#pragma unroll
+  for (int dr = 0; dr < 9; dr++) {
+    const bool bb = (dr == 3) | (dr == 5);  // rounds 6, 7 and 10, 11
+    const fr_t f0 = bb ? sky_bar_t(l, tab) : fr_sqr(l);
+    r = sky_tab_reduce(fr_add_raw(r, f0), tab, 2 * dr);
+    const fr_t f1 = bb ? sky_bar_t(r, tab) : fr_sqr(r);
+    l = sky_tab_reduce(fr_add_raw(l, f1), tab, 2 * dr + 1);
+  }
+  // l + t < 2p + p + 6 2^224: the row-0 reduction leaves it below
+  // p + 6 2^224, one conditional subtraction of p makes it canonical
+  fr_t o = sky_tab_reduce(fr_add_raw(l, t), tab, 0);
+  fr_canon(o);
+  return o;
+}
+
 // SkyscraperHashEngine::hash_many of one message of `chunks` words
 // (provekit/common/src/skyscraper/whir.rs:42-100): acc = m0, then
 // acc = compress(acc, m_k).  Words are read four at a time, one 128-byte line
@@ -159,6 +259,35 @@ __device__ __forceinline__ fr_t sky_hash_message(const fr_t* m, uint64_t chunks)
 #pragma unroll 1
     for (int j = 0; j < n; j++) {
       acc = sky_compress_v(acc, w[0]);
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
+    }
+  }
+  return acc;
+}
+
+// sky_hash_message with the table-form compression.
+__device__ __forceinline__ fr_t sky_hash_message_t(const fr_t* m, uint64_t chunks,
+                                                   const SkyTab& tab) {
+  constexpr int G = 4;
+  fr_t acc = {};
+  for (uint64_t g = 0; g < chunks; g += G) {
+    fr_t w[G] = {};
+#pragma unroll
+    for (int j = 0; j < G; j++)
+      if (g + j < chunks) w[j] = fr_load(m + g + j);
+    int n = chunks - g < (uint64_t)G ? (int)(chunks - g) : G;
+    if (g == 0) {  // m0 starts the fold
+      acc = w[0];
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
+      n -= 1;
+    }
+#pragma unroll 1
+    for (int j = 0; j < n; j++) {
+      acc = sky_compress_t(acc, w[0], tab);
       w[0] = w[1];
       w[1] = w[2];
       w[2] = w[3];

@@ -15,11 +15,16 @@ namespace pk {
 
 namespace {
 
+// Every kernel here runs the table form of the compression (sky_compress_t):
+// the workgroup fills its LDS reduction table before any lane exits.
 __global__ __launch_bounds__(256) void k_compress_many(const fr_t* pairs, fr_t* out, uint64_t n) {
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const fr_t l = fr_load(pairs + 2 * i), r = fr_load(pairs + 2 * i + 1);
-  fr_store(out + i, sky_compress(l, r));
+  fr_store(out + i, sky_compress_t(l, r, tab));
 }
 
 // One lane per message of `chunks` words.  At most 128 VGPRs (four waves
@@ -28,15 +33,21 @@ __global__ __launch_bounds__(256) void k_compress_many(const fr_t* pairs, fr_t* 
 // fits three.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hash_fold(const fr_t* in, uint64_t chunks, uint64_t count,
                                                    fr_t* out) {
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  fr_store(out + i, sky_hash_message(in + i * chunks, chunks));
+  fr_store(out + i, sky_hash_message_t(in + i * chunks, chunks, tab));
 }
 
 __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out, uint64_t n_out) {
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
-  fr_store(out + i, sky_compress(fr_load(in + 2 * i), fr_load(in + 2 * i + 1)));
+  fr_store(out + i, sky_compress_t(fr_load(in + 2 * i), fr_load(in + 2 * i + 1), tab));
 }
 
 // The last levels, n_in <= 128 nodes down to the root, in one wave: lane t
@@ -46,6 +57,9 @@ __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out,
 // latency-bound (one compression per level on the critical path), so a
 // single wave is as fast as any spread and needs no launches.
 __global__ __launch_bounds__(64) void k_merkle_tail(fr_t* nodes, uint64_t n_in) {
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const int t = threadIdx.x;
   bool on = (uint64_t)t < (n_in >> 1);
   fr_t x = {}, y = {};
@@ -57,7 +71,7 @@ __global__ __launch_bounds__(64) void k_merkle_tail(fr_t* nodes, uint64_t n_in) 
   for (uint64_t n = n_in; n > 1; n >>= 1) {
     fr_t v = {};
     if (on) {
-      v = sky_compress(x, y);
+      v = sky_compress_t(x, y, tab);
       fr_store(nodes + off + t, v);
     }
     off += n >> 1;
