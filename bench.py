@@ -71,6 +71,7 @@ class Hip:
         self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
         self.L.hipStreamSynchronize.argtypes = [vp]
         self.L.hipEventDestroy.argtypes = [vp]
+        self.L.hipStreamWaitEvent.argtypes = [vp, vp, ctypes.c_uint]
 
     def _ok(self, rc, what):
         if rc != 0:
@@ -96,6 +97,9 @@ class Hip:
 
     def sync(self, s):
         self._ok(self.L.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def wait(self, s, e):
+        self._ok(self.L.hipStreamWaitEvent(s, e, 0), "hipStreamWaitEvent")
 
 
 def pct(xs, q):
@@ -758,43 +762,68 @@ def main():
     nodes = 2 * leaves - 1
     coeffs = random_fr(n, seed=args.seed, stream="commit-bench")
     d_coeffs = pk.DeviceBuffer.from_array(coeffs)
-    d_cw = pk.DeviceBuffer(32 * N)
-    d_nodes = pk.DeviceBuffer(32 * nodes)
+    # two sets of outputs: successive commits alternate between two streams,
+    # so commit k's latency-bound tree top runs beside commit k+1's NTT
+    s2 = hip.stream()
+    streams = (s, s2)
+    bufs = [(pk.DeviceBuffer(32 * N), pk.DeviceBuffer(32 * nodes)) for _ in range(2)]
 
-    def step(evs=None):
+    def step(k, evs=None, pipelined=True):
+        st = streams[k % 2] if pipelined else s
+        d_cw, d_nodes = bufs[k % 2] if pipelined else bufs[0]
         if evs:
-            hip.record(evs[0], s)
-        pk._call("pkgpu_rs_interleaved_encode_device", d_coeffs.ptr, 1, n, None, 0, N, 1, d_cw.ptr, s)
+            hip.record(evs[0], st)
+        pk._call("pkgpu_rs_interleaved_encode_device", d_coeffs.ptr, 1, n, None, 0, N, 1, d_cw.ptr, st)
         if evs:
-            hip.record(evs[1], s)
-        pk._call("pkgpu_skyscraper_hash_many_device", 32 * args.leaf, d_cw.ptr, leaves, d_nodes.ptr, s)
+            hip.record(evs[1], st)
+        pk._call("pkgpu_skyscraper_hash_many_device", 32 * args.leaf, d_cw.ptr, leaves, d_nodes.ptr, st)
         if evs:
-            hip.record(evs[2], s)
-        pk._call("pkgpu_skyscraper_merkle_device", d_nodes.ptr, leaves, d_nodes.ptr, s)
+            hip.record(evs[2], st)
+        pk._call("pkgpu_skyscraper_merkle_device", d_nodes.ptr, leaves, d_nodes.ptr, st)
         if evs:
-            hip.record(evs[3], s)
+            hip.record(evs[3], st)
 
-    for _ in range(args.warmup):
-        step()
+    for k in range(args.warmup):
+        step(k)
+        step(k, pipelined=False)
     hip.sync(s)
+    hip.sync(s2)
     pk.synchronize()
 
     K = args.steps
+    # timed region: K commits, alternating streams; bracketed by events on s
+    # that s2 waits for / joins
     evs = [[hip.event() for _ in range(4)] for _ in range(K)]
+    e_begin, e_join, e_end = hip.event(), hip.event(), hip.event()
     t0 = time.perf_counter()
+    hip.record(e_begin, s)
+    hip.wait(s2, e_begin)
     for k in range(K):
-        step(evs[k])
+        step(k, evs[k])
+    hip.record(e_join, s2)
+    hip.wait(s, e_join)
+    hip.record(e_end, s)
     hip.sync(s)
     pk.synchronize()
     wall = time.perf_counter() - t0
-
-    lat = [hip.elapsed_ms(e[0], e[3]) for e in evs]
-    ntt = [hip.elapsed_ms(e[0], e[1]) for e in evs]
-    leaf = [hip.elapsed_ms(e[1], e[2]) for e in evs]
-    tree = [hip.elapsed_ms(e[2], e[3]) for e in evs]
-    gpu_total_ms = hip.elapsed_ms(evs[0][0], evs[-1][3])
+    gpu_total_ms = hip.elapsed_ms(e_begin, e_end)
     ms_per_step = gpu_total_ms / K
     value = n * K / (gpu_total_ms / 1e3)
+    lat = [hip.elapsed_ms(e[0], e[3]) for e in evs]  # each commit's own chain, under the overlap
+
+    # the same commits one after another on one stream: per-stage times of
+    # each kernel alone (the roofline's launch durations) and unloaded latency
+    evs1 = [[hip.event() for _ in range(4)] for _ in range(K)]
+    for k in range(K):
+        step(k, evs1[k], pipelined=False)
+    hip.sync(s)
+    lat_serial = [hip.elapsed_ms(e[0], e[3]) for e in evs1]
+    ntt = [hip.elapsed_ms(e[0], e[1]) for e in evs1]
+    leaf = [hip.elapsed_ms(e[1], e[2]) for e in evs1]
+    tree = [hip.elapsed_ms(e[2], e[3]) for e in evs1]
+    serial_ms = hip.elapsed_ms(evs1[0][0], evs1[-1][3]) / K
+    d_cw, d_nodes = bufs[(K - 1) % 2]
+    root_other = bufs[K % 2][1].download(np.uint8, nbytes=32 * nodes)[-32:].tobytes()
 
     root = d_nodes.download(np.uint8, nbytes=32 * nodes)[-32:].tobytes()
 
@@ -858,11 +887,17 @@ def main():
         },
         "p50_commit_ms": pct(lat, 0.5),
         "p99_commit_ms": pct(lat, 0.99),
+        "pipelining": "commits alternate between two streams (commit k's tree top beside commit k+1's NTT); "
+                      "p50/p99 from each commit's own events",
+        "serial_ms_per_commit": serial_ms,
+        "serial_p50_commit_ms": pct(lat_serial, 0.5),
+        "serial_p99_commit_ms": pct(lat_serial, 0.99),
         "stages_ms": {"rs_ntt": ntt_ms, "leaf_hash": leaf_ms, "merkle_tree": tree_ms},
         "codeword_elements_per_s": N * K / (gpu_total_ms / 1e3),
         "skyscraper_hashes_per_s": (compressions_leaf + leaves - 1) * K / (gpu_total_ms / 1e3),
         "wall_s": wall,
         "root": root.hex(),
+        "roots_equal_across_streams": root == root_other,
         "roofline": roofline,
         "roofline_ntt": roofline_ntt,
     }

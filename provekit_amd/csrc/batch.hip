@@ -68,6 +68,10 @@ constexpr int kQueueBlock = 256;
 __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_hash_queue(
     const BatchPolyDev* polys, const HashTask* tasks, uint32_t ntasks, uint32_t* done,
     uint32_t* next, uint32_t* err) {
+  // the compression's reduction table, filled before the waves go their own way
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const int t = threadIdx.x & (kQueueWave - 1);
   for (;;) {
     const uint32_t id = wave_take(next);
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
         on = (uint64_t)t < (n >> 1);
       }
       fr_t v = {};
-      if (on) v = sky_compress(x, y);
+      if (on) v = sky_compress_t(x, y, tab);
       if (kind == kTaskLeaf) {
         acc = v;
       } else if (kind == kTaskNode) {

@@ -34,6 +34,9 @@ __device__ __forceinline__ bool lt256(const fr_t& a, const fr_t& b) {
 __global__ __launch_bounds__(256) void k_pow_window(fr_t challenge, fr_t thr, uint64_t base,
                                                     uint64_t count,
                                                     unsigned long long* best) {
+  __shared__ SkyTab tab;
+  sky_tab_fill(tab);
+  __syncthreads();
   const uint64_t first = base + (uint64_t)blockIdx.x * blockDim.x;
   if (first > __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint64_t off = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,7 +47,7 @@ __global__ __launch_bounds__(256) void k_pow_window(fr_t challenge, fr_t thr, ui
   for (int i = 0; i < 8; i++) r.l[i] = 0;
   r.l[0] = (uint32_t)nonce;
   r.l[1] = (uint32_t)(nonce >> 32);
-  const fr_t h = sky_compress(challenge, r);
+  const fr_t h = sky_compress_t(challenge, r, tab);
   if (lt256(h, thr)) atomicMin(best, (unsigned long long)nonce);
 }
 

@@ -41,7 +41,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   fr_store(out + i, sky_hash_message_t(in + i * chunks, chunks, tab));
 }
 
+// The tree's upper levels are a chain of latency-bound launches (one
+// compression per level on the critical path).  When another commit's
+// kernels share the SIMDs (commits pipelined on two streams), their waves
+// would take most issue slots from these few: the tree waves raise their
+// issue priority (s_setprio) so the chain runs at close to its lone-wave
+// speed while the other work fills the remaining slots.
 __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out, uint64_t n_out) {
+  __builtin_amdgcn_s_setprio(3);
   __shared__ SkyTab tab;
   sky_tab_fill(tab);
   __syncthreads();
@@ -57,6 +64,7 @@ __global__ __launch_bounds__(256) void k_merkle_level(const fr_t* in, fr_t* out,
 // latency-bound (one compression per level on the critical path), so a
 // single wave is as fast as any spread and needs no launches.
 __global__ __launch_bounds__(64) void k_merkle_tail(fr_t* nodes, uint64_t n_in) {
+  __builtin_amdgcn_s_setprio(3);
   __shared__ SkyTab tab;
   sky_tab_fill(tab);
   __syncthreads();
