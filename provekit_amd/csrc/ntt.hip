@@ -110,6 +110,21 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   const uint64_t base = h * (Q << a.r) + j0;
   const uint32_t jmask = (1u << jb) - 1;
 
+  // Twiddle of butterfly k of this thread in stage t: rr[(h << t) | (w0 >> (r - t))].
+  // Each stage's twiddles are loaded one stage ahead (the loads stay in
+  // flight across the barrier, which waits on LDS only), so the global load
+  // latency hides behind the previous stage's products instead of opening
+  // every stage.
+  auto tw_index = [&](int t, int k) -> uint64_t {
+    const int pos = a.r - 1 - t;
+    const int wb = (tid + k * T) >> jb;
+    const int w0 = ((wb >> pos) << (pos + 1)) | (wb & ((1 << pos) - 1));
+    return (h << t) | ((uint64_t)w0 >> (a.r - t));
+  };
+  fr_t tw[BPT];
+#pragma unroll
+  for (int k = 0; k < BPT; k++) tw[k] = fr_load(a.rr + tw_index(0, k));
+
   for (int e = tid; e < N; e += T) {
     const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
     fr_t v;
@@ -125,11 +140,17 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   }
   __syncthreads();
 
-  for (int t = 0; t < a.r; t++) {
+  // stage t with its twiddles in cur; the next stage's go into nxt (two
+  // alternating register sets, so nothing is copied between stages)
+  auto stage = [&](int t, fr_t (&cur)[BPT], fr_t (&nxt)[BPT]) {
     const int pos = a.r - 1 - t;
     const int span = 1 << (pos + jb);
-    fr_t tw[BPT], x[BPT], y[BPT];
+    fr_t x[BPT], y[BPT];
     int e0[BPT];
+    if (t + 1 < a.r) {
+#pragma unroll
+      for (int k = 0; k < BPT; k++) nxt[k] = fr_load(a.rr + tw_index(t + 1, k));
+    }
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
       const int b = tid + k * T;
@@ -137,20 +158,23 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
       const int wb = b >> jb;
       const int w0 = ((wb >> pos) << (pos + 1)) | (wb & ((1 << pos) - 1));
       e0[k] = (w0 << jb) | jl;
-      const uint64_t kt = (h << t) | ((uint64_t)w0 >> (a.r - t));
-      tw[k] = fr_load(a.rr + kt);
       x[k] = lds[e0[k]];
       y[k] = lds[e0[k] + span];
     }
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
-      const fr_t m = fr_mul(tw[k], y[k]);
+      const fr_t m = fr_mul(cur[k], y[k]);
       fr_t u = x[k];
       cond_sub<2>(u);
       lds[e0[k]] = fr_add_raw(u, m);
       lds[e0[k] + span] = fr_sub_plus_2p(u, m);
     }
     __syncthreads();
+  };
+  fr_t tw2[BPT];
+  for (int t = 0; t < a.r; t += 2) {
+    stage(t, tw, tw2);
+    if (t + 1 < a.r) stage(t + 1, tw2, tw);
   }
 
   for (int e = tid; e < N; e += T) {
