@@ -197,6 +197,21 @@ int pkgpu_sparse_mul(size_t num_rows, size_t num_cols, const uint32_t* row_start
 int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
                             size_t nnz, const uint32_t* cols, const uint64_t* vals,
                             const uint64_t* z, uint64_t* out, void* stream);
+/* The same product on the reference's own storage: HydratedSparseMatrix is
+ * a SparseMatrix whose entries hold InternedFieldElement ids into an
+ * Interner (sparse_matrix.rs:56-71,583-624, interner.rs:7-42).  ids: nnz u32
+ * indices into interner (interner_len Fr, Montgomery form).  The host form
+ * fails with the reference's "Value not in interner." for an id past the
+ * table; the device form trusts the ids (one past the table reads as zero,
+ * never out of bounds).  Streams 4 bytes per entry instead of 32. */
+int pkgpu_sparse_mul_interned(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                              size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                              const uint64_t* interner, size_t interner_len, const uint64_t* z,
+                              size_t z_len, uint64_t* out);
+int pkgpu_sparse_mul_interned_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                     size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                                     const uint64_t* interner, size_t interner_len,
+                                     const uint64_t* z, uint64_t* out, void* stream);
 /* No counterpart in the reference: pkgpu_sparse_mul_device for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments and results; its

@@ -82,6 +82,8 @@ SIGNATURES = {
     "pkgpu_sparse_mul": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp],
     "pkgpu_sparse_mul_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_sparse_mul_device_noalloc": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sparse_mul_interned": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _sz, _vp],
+    "pkgpu_sparse_mul_interned_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, _vp],
     "pkgpu_interner_decode_device": [_vp, _sz, _vp, ctypes.POINTER(_sz), _vp],
     "pkgpu_sparse_matrix_decode_device": [_vp, _sz, _vp, _sz, _vp, _vp],
     "pkgpu_r1cs_decode_device": [_vp, _sz, ctypes.POINTER(_sz), _vp, ctypes.POINTER(_sz), _vp, _vp],
@@ -587,6 +589,23 @@ def calculate_witness_bounds(A: SparseMatrix, B: SparseMatrix, witness):
     _call("pkgpu_calculate_witness_bounds", A.num_rows, A.num_cols, ra, na, ca, va, rb, nb, cb, vb,
           _ptr(w), w.shape[0], _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]), ctypes.byref(plen))
     return tuple(o[: plen.value] for o in outs)
+
+
+def sparse_mul_interned(num_rows: int, num_cols: int, row_starts, cols, ids, interner, z) -> np.ndarray:
+    """HydratedSparseMatrix * &[Fr] on interned storage (pkgpu_sparse_mul_interned):
+    entry i's value is interner[ids[i]]."""
+    _ensure_init()
+    rs = np.ascontiguousarray(row_starts, dtype=np.uint32)
+    cs = np.ascontiguousarray(cols, dtype=np.uint32)
+    ii = np.ascontiguousarray(ids, dtype=np.uint32)
+    tab = _fr_array(interner) if len(interner) else np.zeros((0, 4), np.uint64)
+    zz = _fr_array(z) if len(z) else np.zeros((0, 4), np.uint64)
+    out = np.zeros((num_rows, 4), dtype=np.uint64)
+    _call("pkgpu_sparse_mul_interned", num_rows, num_cols, _ptr(rs) if rs.size else None, cs.shape[0],
+          _ptr(cs) if cs.size else None, _ptr(ii) if ii.size else None,
+          _ptr(tab) if tab.size else None, tab.shape[0], _ptr(zz) if zz.size else None, zz.shape[0],
+          _ptr(out) if out.size else None)
+    return out
 
 
 def eq_evaluations(r, num_entries: int) -> np.ndarray:

@@ -695,6 +695,56 @@ int pkgpu_sparse_mul_device(size_t num_rows, size_t num_cols, const uint32_t* ro
                  stream_or_thread(stream));
   });
 }
+int pkgpu_sparse_mul_interned(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                              size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                              const uint64_t* interner, size_t interner_len, const uint64_t* z,
+                              size_t z_len, uint64_t* out) {
+  return guarded([&] {
+    REQ(num_cols == z_len, pk::PK_ERR_SIZE, "Vector length does not match number of columns.");
+    REQ(num_rows == 0 || row_starts, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+    REQ(nnz == 0 || (cols && ids), pk::PK_ERR_ARG, "sparse matrix: null entries");
+    REQ(interner_len == 0 || interner, pk::PK_ERR_ARG, "sparse_mul: null interner");
+    REQ(num_rows == 0 || out, pk::PK_ERR_ARG, "sparse_mul: null output");
+    REQ(z_len == 0 || z, pk::PK_ERR_ARG, "sparse_mul: null vector");
+    validate_csr(num_rows, num_cols, row_starts, nnz, cols);
+    for (size_t i = 0; i < nnz; i++)
+      REQ(ids[i] < interner_len, pk::PK_ERR_ARG, "Value not in interner.");  // iter_row's expect
+    if (!num_rows) return;
+    hipStream_t s = pk::thread_stream();
+    Staged drs(4 * num_rows, s), dc(4 * std::max<size_t>(nnz, 1), s),
+        di(4 * std::max<size_t>(nnz, 1), s), dt(sizeof(fr_t) * std::max<size_t>(interner_len, 1), s),
+        dz(sizeof(fr_t) * std::max<size_t>(z_len, 1), s), dout(sizeof(fr_t) * num_rows, s);
+    h2d(drs.get<void>(), row_starts, 4 * num_rows, s);
+    h2d(dc.get<void>(), cols, 4 * nnz, s);
+    h2d(di.get<void>(), ids, 4 * nnz, s);
+    h2d(dt.get<void>(), interner, sizeof(fr_t) * interner_len, s);
+    h2d(dz.get<void>(), z, sizeof(fr_t) * z_len, s);
+    pk::CsrDev A = csr_view(num_rows, num_cols, drs.get<uint32_t>(), nnz, dc.get<uint32_t>(), nullptr);
+    A.ids = di.get<uint32_t>();
+    A.interner = dt.get<fr_t>();
+    A.interner_len = interner_len;
+    pk::spmv_dev(A, dz.get<fr_t>(), dout.get<fr_t>(), s);
+    d2h(out, dout.get<void>(), sizeof(fr_t) * num_rows, s);
+    sync(s);
+  });
+}
+int pkgpu_sparse_mul_interned_device(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                     size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                                     const uint64_t* interner, size_t interner_len,
+                                     const uint64_t* z, uint64_t* out, void* stream) {
+  return guarded([&] {
+    REQ(num_rows == 0 || row_starts, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+    REQ(nnz == 0 || (cols && ids), pk::PK_ERR_ARG, "sparse matrix: null entries");
+    REQ(interner_len == 0 || interner, pk::PK_ERR_ARG, "sparse_mul: null interner");
+    REQ(num_rows == 0 || (out && (num_cols == 0 || z)), pk::PK_ERR_ARG, "sparse_mul: null pointer");
+    pk::CsrDev A = csr_view(num_rows, num_cols, row_starts, nnz, cols, nullptr);
+    A.ids = ids;
+    A.interner = reinterpret_cast<const fr_t*>(interner);
+    A.interner_len = interner_len;
+    pk::spmv_dev(A, reinterpret_cast<const fr_t*>(z), reinterpret_cast<fr_t*>(out),
+                 stream_or_thread(stream));
+  });
+}
 int pkgpu_sparse_mul_device_noalloc(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
                                     size_t nnz, const uint32_t* cols, const uint64_t* vals,
                                     const uint64_t* z, uint64_t* out, void* stream) {

@@ -61,9 +61,31 @@ __device__ __forceinline__ void row_range(const uint32_t* row_starts, uint64_t r
   b = min((uint64_t)row_starts[r], e);
 }
 
+// Where entry i's value comes from: hydrated (one Fr per entry, 32 bytes
+// streamed per entry) or interned, as the reference stores a SparseMatrix
+// (sparse_matrix.rs:56-71: an id per entry into the Interner's table,
+// interner.rs:7-42, hydrated at use, :583-592): 4 bytes per entry and a
+// table that real constraint systems keep small (a few distinct
+// coefficients), so it stays in cache.  An id past the table reads as zero
+// (device data is not validated; never an out-of-bounds read).
+struct ValsHydrated {
+  const fr_t* v;
+  __device__ __forceinline__ fr_t operator()(uint64_t i) const { return fr_load(v + i); }
+};
+struct ValsInterned {
+  const uint32_t* id;
+  const fr_t* table;
+  uint64_t n;
+  __device__ __forceinline__ fr_t operator()(uint64_t i) const {
+    const uint32_t k = id[i];
+    return k < n ? fr_load(table + k) : fr_zero();
+  }
+};
+
 // sum of v[i] z[col[i]] over [b, e), one lane
+template <class V>
 __device__ __forceinline__ fr_t row_dot(uint64_t b, uint64_t e, uint64_t ncols, const uint32_t* cols,
-                                        const fr_t* vals, const fr_t* z) {
+                                        const V& vals, const fr_t* z) {
   fr_t acc = fr_zero();
   // take the entries four at a time, with all index, value and gather loads
   // issued before the products, so a lane keeps several independent HBM
@@ -77,7 +99,7 @@ __device__ __forceinline__ fr_t row_dot(uint64_t b, uint64_t e, uint64_t ncols, 
 #pragma unroll
     for (int k = 0; k < G; k++) {
       if (c[k] < ncols) {
-        v[k] = fr_load(vals + i0 + k);
+        v[k] = vals(i0 + k);
         x[k] = fr_load(z + c[k]);
       }
     }
@@ -89,9 +111,10 @@ __device__ __forceinline__ fr_t row_dot(uint64_t b, uint64_t e, uint64_t ncols, 
 }
 
 // spans: [0] = count, then the spanning long rows (capacity n_chunks)
+template <class V>
 __global__ __launch_bounds__(256) void k_spmv(uint64_t rows, uint64_t ncols,
                                               const uint32_t* row_starts, uint64_t nnz,
-                                              const uint32_t* cols, const fr_t* vals,
+                                              const uint32_t* cols, V vals,
                                               const fr_t* z, fr_t* out, uint32_t* chunk_flags,
                                               uint32_t* spans, uint64_t n_chunks) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,9 +149,10 @@ __device__ __forceinline__ fr_t block_sum1(fr_t v) {
   return s;
 }
 
+template <class V>
 __global__ __launch_bounds__(256) void k_spmv_chunks(uint64_t rows, uint64_t ncols,
                                                      const uint32_t* row_starts, uint64_t nnz,
-                                                     const uint32_t* cols, const fr_t* vals,
+                                                     const uint32_t* cols, V vals,
                                                      const fr_t* z, fr_t* out,
                                                      const uint32_t* chunk_flags, fr_t* head,
                                                      fr_t* tail, uint64_t n_chunks) {
@@ -175,7 +199,7 @@ __global__ __launch_bounds__(256) void k_spmv_chunks(uint64_t rows, uint64_t nco
       fr_t acc = fr_zero();
       for (uint64_t i = lo + t; i < hi; i += blockDim.x) {
         const uint32_t cc = cols[i];
-        if (cc < ncols) acc = fr_add(acc, fr_mul(fr_load(vals + i), fr_load(z + cc)));
+        if (cc < ncols) acc = fr_add(acc, fr_mul(vals(i), fr_load(z + cc)));
       }
       fr_t sum = block_sum1(acc);
       if (t == 0) {
@@ -745,21 +769,30 @@ __global__ __launch_bounds__(kTailThreads) void k_sc_tail(ScState* st, fr_t* a, 
 // ------------------------------------------------------------------ launchers
 uint64_t spmv_chunks(uint64_t nnz) { return nnz ? (nnz + kChunk - 1) / kChunk : 1; }
 
-void spmv_launch(const CsrDev& A, const fr_t* z, fr_t* out, uint32_t* flags, fr_t* partials,
-                 hipStream_t s) {
-  if (!A.rows) return;
+template <class V>
+static void spmv_launch_v(const CsrDev& A, V vals, const fr_t* z, fr_t* out, uint32_t* flags,
+                          fr_t* partials, hipStream_t s) {
   const uint64_t n_chunks = spmv_chunks(A.nnz);
   // flags[n_chunks], spans[1 + n_chunks], head/tail partials[n_chunks]
   uint32_t* spans = flags + n_chunks;
   fr_t* head = partials;
   fr_t* tail = head + n_chunks;
   k_spmv_clear<<<nblk(n_chunks + 1, 256), 256, 0, s>>>(flags, n_chunks + 1);  // flags and the count
-  k_spmv<<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals,
-                                            z, out, flags, spans, n_chunks);
-  k_spmv_chunks<<<(unsigned)std::min<uint64_t>(n_chunks, 1024), 256, 0, s>>>(
-      A.rows, A.ncols, A.row_starts, A.nnz, A.cols, A.vals, z, out, flags, head, tail, n_chunks);
+  k_spmv<V><<<nblk(A.rows, 256), 256, 0, s>>>(A.rows, A.ncols, A.row_starts, A.nnz, A.cols, vals,
+                                               z, out, flags, spans, n_chunks);
+  k_spmv_chunks<V><<<(unsigned)std::min<uint64_t>(n_chunks, 1024), 256, 0, s>>>(
+      A.rows, A.ncols, A.row_starts, A.nnz, A.cols, vals, z, out, flags, head, tail, n_chunks);
   k_spmv_spans<<<64, 256, 0, s>>>(A.rows, A.row_starts, A.nnz, spans, n_chunks, head, tail, out);
   PK_HIP(hipGetLastError());
+}
+
+void spmv_launch(const CsrDev& A, const fr_t* z, fr_t* out, uint32_t* flags, fr_t* partials,
+                 hipStream_t s) {
+  if (!A.rows) return;
+  if (A.ids)
+    spmv_launch_v(A, ValsInterned{A.ids, A.interner, A.interner_len}, z, out, flags, partials, s);
+  else
+    spmv_launch_v(A, ValsHydrated{A.vals}, z, out, flags, partials, s);
 }
 
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s) {

@@ -44,7 +44,12 @@ struct CsrDev {
   uint64_t rows, ncols, nnz;
   const uint32_t* row_starts;  // rows entries; row r ends where row r+1 starts (last: nnz)
   const uint32_t* cols;
-  const fr_t* vals;
+  const fr_t* vals;  // hydrated values (nnz), or null when ids are set
+  // interned values: ids (nnz) into the interner table (interner_len Fr);
+  // only the SpMV reads this form
+  const uint32_t* ids = nullptr;
+  const fr_t* interner = nullptr;
+  uint64_t interner_len = 0;
 };
 
 void spmv_dev(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s);

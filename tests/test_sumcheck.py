@@ -371,6 +371,42 @@ def test_gpu_sparse_mul_device_malformed_stays_in_bounds(gpu, oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("rows,n_values", [(1, 1), (1000, 16), (1 << 16, 3), (1 << 18, 5000)])
+def test_gpu_sparse_mul_interned(gpu, oracle, rows, n_values):
+    """The interned form (ids into an interner, the reference's storage)
+    against the oracle on the hydrated matrix, with long rows (chunked path)."""
+    rng = np.random.default_rng(rows)
+    cols = max(rows, 4096)
+    rs, cs, _ = random_csr(rng, rows, cols, 5)
+    if rows >= 1000:  # a few long rows across chunks
+        extra = rng.choice(cols, size=3000, replace=False)
+        cs = np.concatenate([cs[: rs[1]], np.sort(extra).astype(np.uint32), cs[rs[1]:]])
+        rs = rs.copy()
+        rs[1:] += 3000
+    interner = random_fr(n_values, seed=n_values, stream="interner")
+    ids = rng.integers(0, n_values, size=cs.shape[0]).astype(np.uint32)
+    z = random_fr(cols, seed=rows, stream="interned-z")
+    print(f"[case] interned spmv rows={rows} values={n_values} nnz={cs.shape[0]}", flush=True)
+    got = gpu.sparse_mul_interned(rows, cols, rs, cs, ids, interner, z)
+    exp = oracle.sparse_mul(rows, cols, rs, cs, np.ascontiguousarray(interner[ids]), z)
+    assert np.array_equal(got, exp)
+    # device form, ids past the table read as zero (in bounds)
+    bad = ids.copy()
+    bad[::7] = n_values + 5
+    bufs = [gpu.DeviceBuffer.from_array(x) for x in (rs, cs, bad, interner, z)]
+    out = gpu.DeviceBuffer(32 * rows)
+    gpu._call("pkgpu_sparse_mul_interned_device", rows, cols, bufs[0].ptr, cs.shape[0], bufs[1].ptr,
+              bufs[2].ptr, bufs[3].ptr, n_values, bufs[4].ptr, out.ptr, None)
+    gpu.synchronize()
+    vals0 = np.ascontiguousarray(interner[np.minimum(bad, n_values - 1)])
+    vals0[bad >= n_values] = 0
+    assert np.array_equal(out.download(shape=(rows, 4)), oracle.sparse_mul(rows, cols, rs, cs, vals0, z))
+    with pytest.raises(gpu.ProvekitGpuError) as e:
+        gpu.sparse_mul_interned(rows, cols, rs, cs, bad, interner, z)
+    assert "Value not in interner." in str(e.value)
+
+
+@pytest.mark.gpu
 def test_gpu_sparse_mul_errors_and_kats(gpu, kats):
     import provekit_amd as pk
     A = gpu.SparseMatrix(2, 3, [0, 1], [0, 2], frs([5, 6]))
