@@ -104,23 +104,33 @@ __device__ __forceinline__ fr_t fr_add_ilv(const fr_t& a, const fr_t& b) {
   fr_t r;
   uint32_t d[8];
   uint64_t c;
+  // add k (carry: SGPR pair), subtract-2p k (borrow: vcc), one wait state:
+  // every carry and borrow is read two instructions after it is written
   asm(
       "v_add_co_u32_e64 %0, %16, %17, %25\n\t"
-      "v_addc_co_u32_e64 %1, %16, %18, %26, %16\n\t"
       "v_subrev_co_u32_e32 %8, vcc, 0xe0000002, %0\n\t"
-      "v_addc_co_u32_e64 %2, %16, %19, %27, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %1, %16, %18, %26, %16\n\t"
       "v_subb_co_u32_e32 %9, vcc, %1, %33, vcc\n\t"
-      "v_addc_co_u32_e64 %3, %16, %20, %28, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %2, %16, %19, %27, %16\n\t"
       "v_subb_co_u32_e32 %10, vcc, %2, %34, vcc\n\t"
-      "v_addc_co_u32_e64 %4, %16, %21, %29, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %3, %16, %20, %28, %16\n\t"
       "v_subb_co_u32_e32 %11, vcc, %3, %35, vcc\n\t"
-      "v_addc_co_u32_e64 %5, %16, %22, %30, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %4, %16, %21, %29, %16\n\t"
       "v_subb_co_u32_e32 %12, vcc, %4, %36, vcc\n\t"
-      "v_addc_co_u32_e64 %6, %16, %23, %31, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %5, %16, %22, %30, %16\n\t"
       "v_subb_co_u32_e32 %13, vcc, %5, %37, vcc\n\t"
-      "v_addc_co_u32_e64 %7, %16, %24, %32, %16\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %6, %16, %23, %31, %16\n\t"
       "v_subb_co_u32_e32 %14, vcc, %6, %38, vcc\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %7, %16, %24, %32, %16\n\t"
       "v_subb_co_u32_e32 %15, vcc, %7, %39, vcc\n\t"
+      "s_nop 1\n\t"
       "v_cndmask_b32_e32 %0, %8, %0, vcc\n\t"
       "v_cndmask_b32_e32 %1, %9, %1, vcc\n\t"
       "v_cndmask_b32_e32 %2, %10, %2, vcc\n\t"
@@ -148,23 +158,33 @@ __device__ __forceinline__ fr_t fr_sub_ilv(const fr_t& a, const fr_t& b) {
   fr_t r;
   uint32_t u[8];
   uint64_t c;
+  // subtract k (borrow: SGPR pair), add-2p k (carry: vcc), one wait state,
+  // as in fr_add_ilv
   asm(
       "v_sub_co_u32_e64 %0, %16, %17, %25\n\t"
-      "v_subb_co_u32_e64 %1, %16, %18, %26, %16\n\t"
       "v_add_co_u32_e32 %8, vcc, 0xe0000002, %0\n\t"
-      "v_subb_co_u32_e64 %2, %16, %19, %27, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %1, %16, %18, %26, %16\n\t"
       "v_addc_co_u32_e32 %9, vcc, %33, %1, vcc\n\t"
-      "v_subb_co_u32_e64 %3, %16, %20, %28, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %2, %16, %19, %27, %16\n\t"
       "v_addc_co_u32_e32 %10, vcc, %34, %2, vcc\n\t"
-      "v_subb_co_u32_e64 %4, %16, %21, %29, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %3, %16, %20, %28, %16\n\t"
       "v_addc_co_u32_e32 %11, vcc, %35, %3, vcc\n\t"
-      "v_subb_co_u32_e64 %5, %16, %22, %30, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %4, %16, %21, %29, %16\n\t"
       "v_addc_co_u32_e32 %12, vcc, %36, %4, vcc\n\t"
-      "v_subb_co_u32_e64 %6, %16, %23, %31, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %5, %16, %22, %30, %16\n\t"
       "v_addc_co_u32_e32 %13, vcc, %37, %5, vcc\n\t"
-      "v_subb_co_u32_e64 %7, %16, %24, %32, %16\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %6, %16, %23, %31, %16\n\t"
       "v_addc_co_u32_e32 %14, vcc, %38, %6, vcc\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32_e64 %7, %16, %24, %32, %16\n\t"
       "v_addc_co_u32_e32 %15, vcc, %39, %7, vcc\n\t"
+      "s_nop 0\n\t"
       "v_cndmask_b32_e64 %0, %0, %8, %16\n\t"
       "v_cndmask_b32_e64 %1, %1, %9, %16\n\t"
       "v_cndmask_b32_e64 %2, %2, %10, %16\n\t"
@@ -268,7 +288,7 @@ __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   fr_t r;
   uint64_t c01 = 0;
   uint32_t c2 = 0;
-  uint64_t cy;
+  uint64_t cy[3];
 #include "fr_mul_asm.inc"
   (void)cy;
   return r;
@@ -292,7 +312,7 @@ __device__ __forceinline__ fr_t fr_sqr(const fr_t& a) {
   fr_t r;
   uint64_t c01 = 0;
   uint32_t c2 = 0;
-  uint64_t cy;
+  uint64_t cy[3];
 #include "fr_sqr_asm.inc"
   (void)cy;
   return r;

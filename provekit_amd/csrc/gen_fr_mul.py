@@ -82,33 +82,78 @@ def no_carry_terms(k, square=False):
     return nc
 
 
+# gfx950 (like gfx942) needs two wait states between a VALU instruction that
+# writes an SGPR and a VALU instruction that reads it, the carry-in of an
+# add-with-carry included: hipcc's own code puts `s_nop 1` between the links
+# of a v_add_co / v_addc_co chain for these targets (none for gfx90a).  The
+# carry of a multiply-add is therefore added three instructions later: the
+# carrying terms' carries rotate through three SGPR pairs, so multiply-add i
+# writes pair i % 3 and its carry add follows the multiply-add of term i + 2;
+# an s_nop fills a gap that has fewer than two instructions.
+WAIT_STATES = 2
+CARRY_REGS = 3
+
+
+def column_asm(nc, carrying):
+    """(instructions, operand count used by carry pairs) for one column:
+    %0 is c01, %1 c2, %2.. the carry pairs, then the term operands."""
+    ncar = len(carrying)
+    nreg = CARRY_REGS if ncar else 1
+    base = (2 if ncar else 1) + nreg  # first term operand
+    body = []
+    n = base
+    mad_at = {}  # carrying index -> position of its multiply-add in body
+    cy = lambda i: f"%{(2 if ncar else 1) + i % nreg}"
+
+    def add_carry(i):
+        gap = len(body) - mad_at[i] - 1
+        if gap < WAIT_STATES:
+            body.append(f"s_nop {WAIT_STATES - gap - 1}")
+        if i == 0:
+            # the column's first carry starts c2 (0 + 0 + carry): no zeroing move
+            body.append(f"v_addc_co_u32_e64 %1, {cy(i)}, 0, 0, {cy(i)}")
+        else:
+            body.append(f"v_addc_co_u32_e64 %1, {cy(i)}, 0, %1, {cy(i)}")
+
+    for t in range(len(nc)):
+        # summed without a carry add: the carry-out is never set (see
+        # no_carry_terms) and never read
+        body.append(f"v_mad_u64_u32 %0, {cy(0)}, %{n}, %{n + 1}, %0")
+        n += 2
+    for i in range(ncar):
+        mad_at[i] = len(body)
+        body.append(f"v_mad_u64_u32 %0, {cy(i)}, %{n}, %{n + 1}, %0")
+        n += 2
+        if i >= 2:
+            add_carry(i - 2)
+    for i in range(max(0, ncar - 2), ncar):
+        add_carry(i)
+    return body, nreg
+
+
 def column(k, square=False):
     nc = no_carry_terms(k, square)
     carrying = [t for t in column_terms(k, square) if t not in nc]
-    # operands: %0 c01, then %1 c2 and %2 cy, or only %1 cy when nothing carries
-    cy = "%2" if carrying else "%1"
-    body, ops = [], []
-    n = 3 if carrying else 2
-    for t, (x, y, cls) in enumerate(nc + carrying):
-        body.append(f"v_mad_u64_u32 %0, {cy}, %{n}, %{n + 1}, %0")
-        if t == len(nc):
-            # the column's first carry starts c2 (0 + 0 + carry): no zeroing move
-            body.append("v_addc_co_u32_e64 %1, %2, 0, 0, %2")
-        elif t > len(nc):
-            body.append("v_addc_co_u32_e64 %1, %2, 0, %1, %2")
+    body, nreg = column_asm(nc, carrying)
+    ops = []
+    for (x, y, cls) in nc + carrying:
         ops += [f'"v"({x})', f'"{cls}"({y})']
-        n += 2
-    outs = '"+v"(c01), "=&v"(c2), "=&s"(cy)' if carrying else '"+v"(c01), "=&s"(cy)'
+    if carrying:
+        outs = '"+v"(c01), "=&v"(c2), ' + ", ".join(f'"=&s"(cy[{i}])' for i in range(nreg))
+    else:
+        outs = '"+v"(c01), "=&s"(cy[0])'
     out = []
     if body:
         text = "\\n\\t".join(body)
         out.append(f'  asm("{text}"\n               : {outs}\n'
                    f'               : {", ".join(ops)});')
     if k < 8:
-        # m_k from the column's low word, then m_k * p0 into the column
+        # m_k from the column's low word, then m_k * p0 into the column; the
+        # carry add waits two states for the multiply-add's carry
         out.append(f'  asm("v_lshl_add_u32 %3, %4, 28, %4\\n\\tv_sub_u32 %3, 0, %3\\n\\t'
-                   f'v_mad_u64_u32 %0, %2, %3, %5, %0\\n\\tv_addc_co_u32_e64 %1, %2, 0, %1, %2"\n'
-                   f'               : "+v"(c01), "+v"(c2), "=&s"(cy), "=&v"(m[{k}])\n'
+                   f'v_mad_u64_u32 %0, %2, %3, %5, %0\\n\\ts_nop 1\\n\\t'
+                   f'v_addc_co_u32_e64 %1, %2, 0, %1, %2"\n'
+                   f'               : "+v"(c01), "+v"(c2), "=&s"(cy[0]), "=&v"(m[{k}])\n'
                    f'               : "v"((uint32_t)c01), "s"(p_limb(0)));')
     else:
         out.append(f"  r.l[{k - 8}] = (uint32_t)c01;")
@@ -129,11 +174,11 @@ def write(name, head, square):
 def main():
     write("fr_mul_asm.inc", "The body of fr_mul (fr.hpp):\n"
           "// in a, b (lazy, or one canonical and one < 4p: a_7 + b_7 <= 2^32 - 18);\n"
-          "// out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy declared there.",
+          "// out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy[3] declared there.",
           False)
     write("fr_sqr_asm.inc", "The body of fr_sqr (fr.hpp):\n"
           "// in a (< 2p) and its doubled limbs u, d; out r = a * a * 2^-256 (lazy);\n"
-          "// m[8], c01, c2, cy declared there.", True)
+          "// m[8], c01, c2, cy[3] declared there.", True)
 
 
 if __name__ == "__main__":

@@ -185,3 +185,71 @@ def test_generated_files_are_current(gen, tmp_path, monkeypatch):
         with open(os.path.join(ROOT, "provekit_amd", "csrc", name)) as f:
             committed = f.read()
         assert (tmp_path / name).read_text() == committed, name
+
+
+# ---- wait states: gfx950 needs two between a VALU write of an SGPR (or vcc)
+# and a VALU read of it, a carry-in included (hipcc puts `s_nop 1` between
+# the links of its own add-with-carry chains for gfx942/gfx950)
+def asm_blocks(path):
+    """The instruction lists of every asm statement in a source file, with the
+    operand numbers the statement binds to SGPR outputs ("=&s" / "=s")."""
+    import re
+    src = open(path).read()
+    out = []
+    for m in re.finditer(r'asm\(\s*((?:"[^"]*"\s*)+):([^:;]*)(?::([^:;]*))?', src):
+        text = "".join(re.findall(r'"([^"]*)"', m.group(1)))
+        outs = re.findall(r'"([^"]*)"\s*\(', m.group(2))
+        sgpr = {f"%{i}" for i, c in enumerate(outs) if "s" in c}
+        insts = [t.strip() for t in text.replace("\\n", "\n").replace("\\t", "").split("\n") if t.strip()]
+        out.append((insts, sgpr))
+    return out
+
+
+def sgpr_hazards(insts, sgpr):
+    """(instruction, register, wait states seen) for every read of an SGPR
+    or vcc fewer than two wait states after the VALU that wrote it."""
+    last_write, pos, bad = {}, 0, []
+    for ins in insts:
+        op, _, rest = ins.partition(" ")
+        args = [a.strip() for a in rest.split(",")] if rest else []
+        if op == "s_nop":
+            pos += int(args[0]) + 1
+            continue
+        reads, writes = [], []
+        if op in ("v_addc_co_u32_e64", "v_subb_co_u32_e64"):
+            reads.append(args[4])
+            writes.append(args[1])
+        elif op in ("v_add_co_u32_e64", "v_sub_co_u32_e64", "v_mad_u64_u32"):
+            writes.append(args[1])
+        elif op == "v_cndmask_b32_e64":
+            reads.append(args[3])
+        elif op == "v_cndmask_b32_e32":
+            reads.append("vcc")
+        elif op.endswith("_e32") and ("co_u32" in op or "cndmask" in op):
+            if args[-1] == "vcc" and op not in ("v_add_co_u32_e32", "v_sub_co_u32_e32", "v_subrev_co_u32_e32"):
+                reads.append("vcc")
+            writes.append("vcc")
+        for r in reads:
+            if (r in sgpr or r == "vcc") and r in last_write and pos - last_write[r] - 1 < 2:
+                bad.append((ins, r, pos - last_write[r] - 1))
+        for w in writes:
+            if w in sgpr or w == "vcc":
+                last_write[w] = pos
+        pos += 1
+    return bad
+
+
+@pytest.mark.parametrize("name", ["fr_mul_asm.inc", "fr_sqr_asm.inc", "fr.hpp"])
+def test_carry_reads_wait_two_states(name):
+    blocks = asm_blocks(os.path.join(ROOT, "provekit_amd", "csrc", name))
+    assert blocks
+    for insts, sgpr in blocks:
+        assert sgpr_hazards(insts, sgpr) == [], insts
+
+
+def test_hazard_checker_catches_a_back_to_back_carry():
+    insts = ["v_mad_u64_u32 %0, %2, %3, %4, %0", "v_addc_co_u32_e64 %1, %2, 0, 0, %2"]
+    assert sgpr_hazards(insts, {"%2"}) == [(insts[1], "%2", 0)]
+    assert sgpr_hazards([insts[0], "s_nop 1", insts[1]], {"%2"}) == []
+    vcc = ["v_add_co_u32_e32 %0, vcc, %1, %2", "v_addc_co_u32_e32 %3, vcc, %4, %5, vcc"]
+    assert sgpr_hazards(vcc, set()) == [(vcc[1], "vcc", 0)]
