@@ -238,6 +238,18 @@ int pkgpu_sparse_transpose_device(size_t num_rows, size_t num_cols, const uint32
                                   size_t nnz, const uint32_t* cols, const uint64_t* vals,
                                   uint32_t* t_row_starts, uint32_t* t_cols, uint64_t* t_vals,
                                   void* stream);
+/* SparseMatrix::transpose on the reference's own storage
+ * (provekit/common/src/sparse_matrix.rs:409-449 moves InternedFieldElement
+ * ids, not field elements): the same transpose with the u32 value ids moving
+ * into t_ids (nnz).  The ids are not looked up, so no interner is needed. */
+int pkgpu_sparse_transpose_interned(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                    size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                                    uint32_t* t_row_starts, uint32_t* t_cols, uint32_t* t_ids);
+int pkgpu_sparse_transpose_interned_device(size_t num_rows, size_t num_cols,
+                                           const uint32_t* row_starts, size_t nnz,
+                                           const uint32_t* cols, const uint32_t* ids,
+                                           uint32_t* t_row_starts, uint32_t* t_cols,
+                                           uint32_t* t_ids, void* stream);
 /* One matrix of multiply_transposed_by_eq_alpha
  * (provekit/common/src/utils/sumcheck.rs:203-227): out (t_num_rows Fr) =
  * At * eq(alpha, .), the eq table over the m device-resident elements of
@@ -248,6 +260,13 @@ int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num
                                                  const uint32_t* t_cols, const uint64_t* t_vals,
                                                  const uint64_t* alpha, size_t m, uint64_t* out,
                                                  void* stream);
+/* The same product on an interned At (the reference's at.hydrate(interner) *
+ * eq_alpha, sumcheck.rs:216-224, without the hydrated copy): t_ids index the
+ * interner table of interner_len Fr; an id beyond it reads as zero. */
+int pkgpu_multiply_transposed_by_eq_alpha_interned_device(
+    size_t t_num_rows, size_t t_num_cols, const uint32_t* t_row_starts, size_t nnz,
+    const uint32_t* t_cols, const uint32_t* t_ids, const uint64_t* interner, size_t interner_len,
+    const uint64_t* alpha, size_t m, uint64_t* out, void* stream);
 /* ---------------------------------------------------- R1CS wire format */
 /* Device CSR filled by the decoders below (caller-allocated buffers). */
 typedef struct {

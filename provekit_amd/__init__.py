@@ -91,6 +91,10 @@ SIGNATURES = {
     "pkgpu_sparse_transpose_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "pkgpu_multiply_transposed_by_eq_alpha_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp,
                                                      _vp],
+    "pkgpu_sparse_transpose_interned": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_sparse_transpose_interned_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pkgpu_multiply_transposed_by_eq_alpha_interned_device": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz,
+                                                              _vp, _sz, _vp, _vp],
     "pkgpu_calculate_witness_bounds": [_sz, _sz, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz,
                                        _vp, _vp, _vp, ctypes.POINTER(_sz)],
     "pkgpu_eq_evaluations": [_vp, _sz, _sz, _vp],
@@ -606,6 +610,23 @@ def sparse_mul_interned(num_rows: int, num_cols: int, row_starts, cols, ids, int
           _ptr(tab) if tab.size else None, tab.shape[0], _ptr(zz) if zz.size else None, zz.shape[0],
           _ptr(out) if out.size else None)
     return out
+
+
+def sparse_transpose_interned(num_rows: int, num_cols: int, row_starts, cols, ids):
+    """SparseMatrix::transpose on interned storage (sparse_matrix.rs:409-449):
+    returns (t_row_starts, t_cols, t_ids), the ids moved with their entries."""
+    _ensure_init()
+    rs = np.ascontiguousarray(row_starts, dtype=np.uint32)
+    cs = np.ascontiguousarray(cols, dtype=np.uint32)
+    ii = np.ascontiguousarray(ids, dtype=np.uint32)
+    trs = np.zeros(num_cols, np.uint32)
+    tcs = np.zeros(cs.shape[0], np.uint32)
+    tis = np.zeros(cs.shape[0], np.uint32)
+    _call("pkgpu_sparse_transpose_interned", num_rows, num_cols, _ptr(rs) if rs.size else None,
+          cs.shape[0], _ptr(cs) if cs.size else None, _ptr(ii) if ii.size else None,
+          _ptr(trs) if trs.size else None, _ptr(tcs) if tcs.size else None,
+          _ptr(tis) if tis.size else None)
+    return trs, tcs, tis
 
 
 def eq_evaluations(r, num_entries: int) -> np.ndarray:

@@ -793,6 +793,48 @@ int pkgpu_sparse_transpose_device(size_t num_rows, size_t num_cols, const uint32
                              stream_or_thread(stream));
   });
 }
+int pkgpu_sparse_transpose_interned(size_t num_rows, size_t num_cols, const uint32_t* row_starts,
+                                    size_t nnz, const uint32_t* cols, const uint32_t* ids,
+                                    uint32_t* t_row_starts, uint32_t* t_cols, uint32_t* t_ids) {
+  return guarded([&] {
+    REQ(num_rows == 0 || row_starts, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+    REQ(nnz == 0 || (cols && ids), pk::PK_ERR_ARG, "sparse matrix: null entries");
+    REQ((num_cols == 0 || t_row_starts) && (nnz == 0 || (t_cols && t_ids)), pk::PK_ERR_ARG,
+        "transpose: null output");
+    validate_csr(num_rows, num_cols, row_starts, nnz, cols);
+    if (!num_cols) return;
+    hipStream_t s = pk::thread_stream();
+    Staged drs(4 * std::max<size_t>(num_rows, 1), s), dc(4 * std::max<size_t>(nnz, 1), s),
+        di(4 * std::max<size_t>(nnz, 1), s), trs(4 * num_cols, s),
+        tcs(4 * std::max<size_t>(nnz, 1), s), tis(4 * std::max<size_t>(nnz, 1), s);
+    h2d(drs.get<void>(), row_starts, 4 * num_rows, s);
+    h2d(dc.get<void>(), cols, 4 * nnz, s);
+    h2d(di.get<void>(), ids, 4 * nnz, s);
+    pk::CsrDev A = csr_view(num_rows, num_cols, drs.get<uint32_t>(), nnz, dc.get<uint32_t>(), nullptr);
+    A.ids = di.get<uint32_t>();
+    pk::sparse_transpose_interned_dev(A, trs.get<uint32_t>(), tcs.get<uint32_t>(),
+                                      tis.get<uint32_t>(), s);
+    d2h(t_row_starts, trs.get<void>(), 4 * num_cols, s);
+    d2h(t_cols, tcs.get<void>(), 4 * nnz, s);
+    d2h(t_ids, tis.get<void>(), 4 * nnz, s);
+    sync(s);
+  });
+}
+int pkgpu_sparse_transpose_interned_device(size_t num_rows, size_t num_cols,
+                                           const uint32_t* row_starts, size_t nnz,
+                                           const uint32_t* cols, const uint32_t* ids,
+                                           uint32_t* t_row_starts, uint32_t* t_cols,
+                                           uint32_t* t_ids, void* stream) {
+  return guarded([&] {
+    REQ(num_rows == 0 || row_starts, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+    REQ(nnz == 0 || (cols && ids), pk::PK_ERR_ARG, "sparse matrix: null entries");
+    REQ((num_cols == 0 || t_row_starts) && (nnz == 0 || (t_cols && t_ids)), pk::PK_ERR_ARG,
+        "transpose: null output");
+    pk::CsrDev A = csr_view(num_rows, num_cols, row_starts, nnz, cols, nullptr);
+    A.ids = ids;
+    pk::sparse_transpose_interned_dev(A, t_row_starts, t_cols, t_ids, stream_or_thread(stream));
+  });
+}
 int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num_cols,
                                                  const uint32_t* t_row_starts, size_t nnz,
                                                  const uint32_t* t_cols, const uint64_t* t_vals,
@@ -806,6 +848,26 @@ int pkgpu_multiply_transposed_by_eq_alpha_device(size_t t_num_rows, size_t t_num
         "transposed eq product: null pointer");
     pk::transposed_eq_mul_dev(csr_view(t_num_rows, t_num_cols, t_row_starts, nnz, t_cols, t_vals),
                               reinterpret_cast<const fr_t*>(alpha), (int)m,
+                              reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
+  });
+}
+int pkgpu_multiply_transposed_by_eq_alpha_interned_device(
+    size_t t_num_rows, size_t t_num_cols, const uint32_t* t_row_starts, size_t nnz,
+    const uint32_t* t_cols, const uint32_t* t_ids, const uint64_t* interner, size_t interner_len,
+    const uint64_t* alpha, size_t m, uint64_t* out, void* stream) {
+  return guarded([&] {
+    REQ(t_num_rows == 0 || t_row_starts, pk::PK_ERR_ARG, "sparse matrix: null row starts");
+    REQ(nnz == 0 || (t_cols && t_ids), pk::PK_ERR_ARG, "sparse matrix: null entries");
+    REQ(interner_len == 0 || interner, pk::PK_ERR_ARG, "transposed eq product: null interner");
+    REQ(m < 63 && t_num_cols <= (size_t(1) << m), pk::PK_ERR_SIZE,
+        "assertion failed: num_entries <= full_size");
+    REQ(t_num_rows == 0 || (out && (m == 0 || alpha)), pk::PK_ERR_ARG,
+        "transposed eq product: null pointer");
+    pk::CsrDev At = csr_view(t_num_rows, t_num_cols, t_row_starts, nnz, t_cols, nullptr);
+    At.ids = t_ids;
+    At.interner = reinterpret_cast<const fr_t*>(interner);
+    At.interner_len = interner_len;
+    pk::transposed_eq_mul_dev(At, reinterpret_cast<const fr_t*>(alpha), (int)m,
                               reinterpret_cast<fr_t*>(out), stream_or_thread(stream));
   });
 }

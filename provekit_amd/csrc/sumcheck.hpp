@@ -69,6 +69,10 @@ void spmv_dev_noalloc(const CsrDev& A, const fr_t* z, fr_t* out, hipStream_t s);
 // t_vals A.nnz (entries in (column, row) order).
 void sparse_transpose_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
                           fr_t* t_vals, hipStream_t s);
+// The same transpose of an interned matrix (A.ids set): the ids move with
+// their entries into t_ids (A.nnz), as the reference's transpose moves them.
+void sparse_transpose_interned_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
+                                   uint32_t* t_ids, hipStream_t s);
 // out = At * eq(alpha, .) with the eq table truncated to At.ncols entries
 // (one matrix of multiply_transposed_by_eq_alpha); alpha: m device elements.
 void transposed_eq_mul_dev(const CsrDev& At, const fr_t* alpha, int m, fr_t* out,

@@ -7,8 +7,10 @@ CPU: the oracle's transpose against a transpose written from the definition
 R1CS (sumcheck.rs:355-435: the rows of A recovered through At * e_i, and the
 expected At/Bt/Ct * eq(alpha) vectors), and the involution T(T(A)) = A.
 GPU: the C ABI bit-exact against the oracle on ragged shapes (empty rows and
-columns, one very long row, one very long column, no entries) and at 2^22
-rows with 1-4 entries per row."""
+columns, one very long row, one very long column, no entries, more than 2^22
+columns, three columns) and at 2^22 rows with 1-4 entries per row, for
+hydrated values and for interned ids (the reference's own storage, moved as
+opaque values); the interned At * eq(alpha) against the hydrated one."""
 import numpy as np
 import pytest
 
@@ -93,7 +95,21 @@ def ragged_cases():
                   np.zeros((0, 4), np.uint64)))
     # single entry in the last column
     cases.append(("single", 1, 9) + csr_from_rows([[(5, 8)]]))
+    # more than 2^22 columns: the sort takes three digit passes (8, 8, 7 bits)
+    rs, cs, vs = random_csr(rng, 20000, 5_000_000, (0, 5), seed=2)
+    cases.append(("wide-5M-cols", 20000, 5_000_000, rs, cs, vs))
+    # three columns over many tiles: most lanes of a wave share a digit
+    rs, cs, vs = random_csr(rng, 1 << 17, 3, (1, 4), seed=3)
+    cases.append(("three-cols", 1 << 17, 3, rs, cs, vs))
     return cases
+
+
+def ids_as_fr(ids):
+    """Interned ids as opaque 32-byte values (limb 0 = id), so that the
+    oracle's transpose moves them like field elements."""
+    v = np.zeros((len(ids), 4), np.uint64)
+    v[:, 0] = ids
+    return v
 
 
 # ---------------------------------------------------------------- CPU: oracle
@@ -192,3 +208,53 @@ def test_gpu_multiply_transposed_by_eq_alpha_random(gpu, oracle, rows, m):
     for g, A in zip(got, mats):
         t = oracle.sparse_transpose(rows, cols, A.row_starts, A.cols, A.vals)
         assert np.array_equal(g, oracle.sparse_mul(cols, rows, *t, eqa))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ragged_cases(), ids=lambda c: c[0])
+def test_gpu_transpose_interned_ragged(gpu, oracle, case):
+    name, rows, cols, rs, cs, _ = case
+    ids = np.random.default_rng(len(cs)).integers(0, 1 << 20, size=len(cs)).astype(np.uint32)
+    print(f"[transpose interned] {name} rows={rows} cols={cols} nnz={len(cs)}", flush=True)
+    trs, tcs, tis = gpu.sparse_transpose_interned(rows, cols, rs, cs, ids)
+    want = oracle.sparse_transpose(rows, cols, rs, cs, ids_as_fr(ids))
+    assert np.array_equal(trs, want[0])
+    assert np.array_equal(tcs, want[1])
+    assert np.array_equal(tis, want[2][:, 0].astype(np.uint32))
+
+
+@pytest.mark.gpu
+def test_gpu_transpose_interned_device_full_size(gpu, oracle):
+    rng = np.random.default_rng(22)
+    n = 1 << 22
+    rs, cs, _ = random_csr(rng, n, n, seed=22)
+    nnz = cs.shape[0]
+    ids = rng.integers(0, 1 << 16, size=nnz).astype(np.uint32)
+    print(f"[transpose interned] device 2^22 nnz={nnz}", flush=True)
+    d = [gpu.DeviceBuffer.from_array(x) for x in (rs, cs, ids)]
+    trs, tcs, tis = gpu.DeviceBuffer(4 * n), gpu.DeviceBuffer(4 * nnz), gpu.DeviceBuffer(4 * nnz)
+    gpu._call("pkgpu_sparse_transpose_interned_device", n, n, d[0].ptr, nnz, d[1].ptr, d[2].ptr,
+              trs.ptr, tcs.ptr, tis.ptr, None)
+    want = oracle.sparse_transpose(n, n, rs, cs, ids_as_fr(ids))
+    assert np.array_equal(trs.download(np.uint32), want[0])
+    assert np.array_equal(tcs.download(np.uint32), want[1])
+    assert np.array_equal(tis.download(np.uint32), want[2][:, 0].astype(np.uint32))
+
+
+@pytest.mark.gpu
+def test_gpu_multiply_transposed_by_eq_alpha_interned(gpu, oracle):
+    # the reference's at.hydrate(interner) * eq_alpha on interned storage
+    rng = np.random.default_rng(5)
+    rows, cols, m = 50000, 3000, 16
+    rs, cs, _ = random_csr(rng, rows, cols, seed=5)
+    interner = random_fr(1000, seed=5, stream="interner")
+    ids = rng.integers(0, 1000, size=len(cs)).astype(np.uint32)
+    alpha = random_fr(m, seed=5, stream="alpha")
+    trs, tcs, tis = gpu.sparse_transpose_interned(rows, cols, rs, cs, ids)
+    d = [gpu.DeviceBuffer.from_array(x) for x in (trs, tcs, tis, interner, alpha)]
+    out = gpu.DeviceBuffer(32 * cols)
+    gpu._call("pkgpu_multiply_transposed_by_eq_alpha_interned_device", cols, rows, d[0].ptr, len(tcs),
+              d[1].ptr, d[2].ptr, d[3].ptr, 1000, d[4].ptr, m, out.ptr, None)
+    t = oracle.sparse_transpose(rows, cols, rs, cs, np.ascontiguousarray(interner[ids]))
+    want = oracle.sparse_mul(cols, rows, *t, oracle.eq_evals(alpha, rows))
+    assert np.array_equal(out.download(shape=(cols, 4)), want)
