@@ -708,6 +708,41 @@ def workload_sumcheck(pk, hip, s, args):
                                  pyoracle.sparse_mul(n, n, mats_h[0][0], mats_h[0][1],
                                                      np.ascontiguousarray(tab_h[ids_h[0]]), z)))
     spmv_int_bytes = sum(k * (4 + 4 + 32) for k in nnz) + 3 * n * (4 + 32)
+    # the reference's own flow after the sumcheck on that storage:
+    # SparseMatrix::transpose moves the u32 ids, then at.hydrate(interner) *
+    # eq_alpha (sumcheck.rs:194-227)
+    tids = [(pk.DeviceBuffer(4 * n), pk.DeviceBuffer(4 * k), pk.DeviceBuffer(4 * k)) for k in nnz]
+    iouts = [pk.DeviceBuffer(32 * n) for _ in range(3)]
+
+    def transpose_interned_all():
+        for (rs, cs, _), k, idb, (trs, tcs, tis) in zip(mats, nnz, ids, tids):
+            pk._call("pkgpu_sparse_transpose_interned_device", n, n, rs.ptr, k, cs.ptr, idb.ptr,
+                     trs.ptr, tcs.ptr, tis.ptr, s)
+
+    def at_eq_interned_all():
+        for (trs, tcs, tis), k, out in zip(tids, nnz, iouts):
+            pk._call("pkgpu_multiply_transposed_by_eq_alpha_interned_device", n, n, trs.ptr, k, tcs.ptr,
+                     tis.ptr, interner.ptr, 16, dalpha.ptr, m, out.ptr, s)
+
+    transpose_interned_all()
+    at_eq_interned_all()
+    hip.sync(s)
+    t_itr, t_ieq = [], []
+    for _ in range(3):
+        e0, e1, e2 = hip.event(), hip.event(), hip.event()
+        hip.record(e0, s)
+        transpose_interned_all()
+        hip.record(e1, s)
+        at_eq_interned_all()
+        hip.record(e2, s)
+        hip.sync(s)
+        t_itr.append(hip.elapsed_ms(e0, e1))
+        t_ieq.append(hip.elapsed_ms(e1, e2))
+    itr_ms, ieq_ms = statistics.mean(t_itr), statistics.mean(t_ieq)
+    hyd0 = np.ascontiguousarray(tab_h[ids_h[0]])
+    iat_ok = bool(np.array_equal(iouts[0].download(shape=(n, 4)),
+                                 pyoracle.sparse_mul(n, n, *pyoracle.sparse_transpose(
+                                     n, n, mats_h[0][0], mats_h[0][1], hyd0), eq_h)))
     ms = spmv_ms + sc_ms
     return base_line("R1CS constraints/s (Az, Bz, Cz + full sumcheck, 2^%d)" % m, n / (ms / 1e3),
                      "constraints/s", steps, args.warmup, ms, True, "u32",
@@ -721,7 +756,12 @@ def workload_sumcheck(pk, hip, s, args):
                           "transpose_hbm_frac": sum(72 * k + 8 * n for k in nnz) / (tr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "at_eq_hbm_frac": (spmv_bytes + 3 * 32 * n) / (eq_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "note": "not in value: SparseMatrix::transpose of A, B, C and At/Bt/Ct * eq(alpha) "
-                                  "(whir_r1cs.rs:172-173); algorithmic bytes 72 nnz + 8 n per transpose"},
+                                  "(whir_r1cs.rs:172-173) on hydrated 32-byte values; algorithmic bytes "
+                                  "72 nnz + 8 n per transpose",
+                          "interned": {"transpose_x3_ms": itr_ms, "at_eq_alpha_x3_ms": ieq_ms,
+                                       "matches_oracle": iat_ok,
+                                       "note": "the same on the reference's storage: u32 ids moved by the "
+                                               "transpose, At * eq(alpha) through the 16-entry interner"}},
                       "roofline_spmv": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms / 1e3) / 1e9,
                                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                         "frac": spmv_bytes / (spmv_ms / 1e3) / 1e9 / HBM_PEAK_GBS,

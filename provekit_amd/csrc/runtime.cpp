@@ -46,8 +46,17 @@ Device& device_current() {
   return device_get(id);
 }
 
+// Requests are rounded up to one of eight size classes per power of two (at
+// most 12.5% over), so that calls whose sizes differ slightly (three matrices
+// with nearly equal entry counts) reuse the blocks the pool keeps instead of
+// mapping new memory each time.
 Scratch::Scratch(size_t bytes, hipStream_t st) : s(st) {
   if (bytes == 0) bytes = 16;
+  if (bytes > 4096) {
+    int lg = 63 - __builtin_clzll(bytes);
+    const size_t step = size_t(1) << (lg - 3);
+    bytes = (bytes + step - 1) / step * step;
+  }
   PK_HIP(hipMallocAsync(&p, bytes, s));
 }
 Scratch::~Scratch() {

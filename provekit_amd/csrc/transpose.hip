@@ -13,9 +13,10 @@
 // first pass takes the keys and builds the payloads from the CSR arrays; the
 // last writes the sorted columns, the rows (t_cols) and the third words (the
 // ids, or the permutation that k_gather_vals follows to fetch the 32-byte
-// values in output order, one random read per entry and sequential writes).
+// values in output order, one random read per entry and sequential writes,
+// while it fills the row starts).
 //
-// One pass over tiles of 4096 entries (512 threads, 512 entries a wave):
+// One pass over tiles of 4096 entries (1024 threads, 256 entries a wave):
 //   k_radix_hist     each tile's digit counts, tile-major: H[tile][bin];
 //   k_radix_scan     one workgroup per bin: H[.][bin] becomes the exclusive
 //                    prefix over tiles, T[bin] the bin's total;
@@ -44,6 +45,8 @@
 // Every kernel keeps its accesses in bounds for any input.  The device path
 // trusts the CSR invariants (non-decreasing row starts within nnz, columns
 // below num_cols); a column beyond num_cols sorts by its low bits.
+#include <initializer_list>
+
 #include "kernels.hpp"
 #include "sumcheck.hpp"
 
@@ -53,10 +56,10 @@ namespace {
 
 constexpr int kMaxDigitBits = 8;
 constexpr uint32_t kBins = 1u << kMaxDigitBits;
-constexpr int kThreads = 512;
+constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
-constexpr int kChunks = 8;                        // 64-entry chunks per wave
-constexpr int kPerWave = 64 * kChunks;            // 512
+constexpr int kChunks = 4;                        // 64-entry chunks per wave
+constexpr int kPerWave = 64 * kChunks;            // 256
 constexpr uint64_t kTile = (uint64_t)kWaves * kPerWave;  // 4096
 constexpr int kScanThreads = 256;
 
@@ -237,22 +240,21 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(Src src, Sink sink, 
   {
     // thread b < bins: the tile's count of digit b, its in-tile first slot
     // (x), the bin's first output slot, and the waves' first slots
-    uint32_t c[kWaves], sum = 0;
+    const bool own = (uint32_t)t < bins;
+    uint32_t sum = 0;
 #pragma unroll
-    for (int v = 0; v < kWaves; v++) {
-      c[v] = (uint32_t)t < bins ? cnt[v][t] : 0u;
-      sum += c[v];
-    }
+    for (int v = 0; v < kWaves; v++) sum += own ? cnt[v][t & (kBins - 1)] : 0u;
     uint32_t total;
     const uint32_t x = block_excl_scan<kThreads>(sum, &total);
-    const uint32_t below_bins = block_excl_scan<kThreads>((uint32_t)t < bins ? T[t] : 0u, &total);
-    if ((uint32_t)t < bins) {
+    const uint32_t below_bins = block_excl_scan<kThreads>(own ? T[t & (kBins - 1)] : 0u, &total);
+    if (own) {
       first[t] = below_bins + H[tile * bins + t] - x;
-      uint32_t s = x;
+      uint32_t run = x;
 #pragma unroll
       for (int v = 0; v < kWaves; v++) {
-        cnt[v][t] = s;
-        s += c[v];
+        const uint32_t c = cnt[v][t];
+        cnt[v][t] = run;
+        run += c;
       }
     }
   }
@@ -291,12 +293,20 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint3
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_gather_vals(const uint32_t* perm, const fr_t* vals,
-                                                     uint64_t n, fr_t* t_vals) {
+// hydrated last step, thread k <= nnz: t_vals[k] = vals[perm[k]] and the
+// row starts that slot k opens (as k_row_starts)
+__global__ __launch_bounds__(256) void k_gather_vals(const uint32_t* perm, const uint32_t* sorted_cols,
+                                                     const fr_t* vals, uint64_t n, uint64_t ncols,
+                                                     fr_t* t_vals, uint32_t* t_row_starts) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t e = perm[k];
-  fr_store(t_vals + k, fr_load(vals + (e < n ? e : 0)));
+  if (k > n) return;
+  if (k < n) {
+    const uint32_t e = perm[k];
+    fr_store(t_vals + k, fr_load(vals + (e < n ? e : 0)));
+  }
+  const uint64_t lo = k == 0 ? 0 : (uint64_t)sorted_cols[k - 1] + 1;
+  const uint64_t hi = k == n ? ncols : (uint64_t)sorted_cols[k] + 1;
+  for (uint64_t c = lo; c < hi && c < ncols; c++) t_row_starts[c] = (uint32_t)k;
 }
 
 struct Plan {
@@ -316,9 +326,25 @@ void radix_pass(const Plan& pl, int p, Src src, Sink sink, uint32_t* H, uint32_t
   PK_HIP(hipGetLastError());
 }
 
-// Stable sort of the entries by column; the last pass writes through `last`
-// (whose keys array receives the sorted columns).
-void sort_by_column(const CsrDev& A, const uint32_t* third, SinkOut last, hipStream_t s) {
+// Sub-buffers carved from one scratch allocation (256-byte aligned).
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(uint64_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* r = reinterpret_cast<T*>(base + off);
+    off += sizeof(T) * (count ? count : 1);
+    return r;
+  }
+  static size_t size(std::initializer_list<uint64_t> bytes) {
+    size_t t = 0;
+    for (uint64_t b : bytes) t += ((b ? b : 1) + 255) & ~size_t(255);
+    return t;
+  }
+};
+
+Plan make_plan(const CsrDev& A) {
   Plan pl;
   pl.n = A.nnz;
   pl.tiles = (pl.n + kTile - 1) / kTile;
@@ -326,21 +352,34 @@ void sort_by_column(const CsrDev& A, const uint32_t* third, SinkOut last, hipStr
   while (pl.bits < 32 && (1ull << pl.bits) < A.ncols) pl.bits++;
   pl.passes = (pl.bits + kMaxDigitBits - 1) / kMaxDigitBits;
   pl.digit = (pl.bits + pl.passes - 1) / pl.passes;
+  return pl;
+}
+
+// scratch bytes of sort_by_column: row_of, H, T, and the ping-pong keys and
+// payloads
+size_t sort_bytes(const Plan& pl) {
   const uint64_t na = pl.passes > 1 ? pl.n : 1, nb = pl.passes > 2 ? pl.n : 1;
-  Scratch row_of(4 * pl.n, s), H(4 * pl.tiles * kBins, s), T(4 * kBins, s);
-  Scratch keys_a(4 * na, s), pays_a(8 * na, s), keys_b(4 * nb, s), pays_b(8 * nb, s);
+  return Carve::size({4 * pl.n, 4 * pl.tiles * kBins, 4 * kBins, 4 * na, 8 * na, 4 * nb, 8 * nb});
+}
+
+// Stable sort of the entries by column; the last pass writes through `last`
+// (whose keys array receives the sorted columns).  ws: sort_bytes(pl) bytes.
+void sort_by_column(const CsrDev& A, const Plan& pl, const uint32_t* third, SinkOut last, Carve ws,
+                    hipStream_t s) {
+  const uint64_t na = pl.passes > 1 ? pl.n : 1, nb = pl.passes > 2 ? pl.n : 1;
+  uint32_t* row_of = ws.take<uint32_t>(pl.n);
+  uint32_t* h = ws.take<uint32_t>(pl.tiles * kBins);
+  uint32_t* tt = ws.take<uint32_t>(kBins);
+  SinkArr cur{ws.take<uint32_t>(na), ws.take<uint2>(na)};
+  SinkArr other{ws.take<uint32_t>(nb), ws.take<uint2>(nb)};
   if (A.rows)
-    k_row_of<<<nblocks(A.rows, 256), 256, 0, s>>>(A.row_starts, A.rows, pl.n, row_of.as<uint32_t>());
+    k_row_of<<<nblocks(A.rows, 256), 256, 0, s>>>(A.row_starts, A.rows, pl.n, row_of);
   PK_HIP(hipGetLastError());
-  const SrcCsr csr{A.cols, row_of.as<uint32_t>(), third};
-  uint32_t* h = H.as<uint32_t>();
-  uint32_t* tt = T.as<uint32_t>();
+  const SrcCsr csr{A.cols, row_of, third};
   if (pl.passes == 1) {
     radix_pass(pl, 0, csr, last, h, tt, s);
     return;
   }
-  SinkArr cur{keys_a.as<uint32_t>(), pays_a.as<uint2>()};
-  SinkArr other{keys_b.as<uint32_t>(), pays_b.as<uint2>()};
   radix_pass(pl, 0, csr, cur, h, tt, s);
   for (int p = 1; p + 1 < pl.passes; p++) {
     radix_pass(pl, p, SrcArr{cur.keys, cur.pays}, other, h, tt, s);
@@ -372,19 +411,28 @@ bool transpose_trivial(const CsrDev& A, uint32_t* t_row_starts, hipStream_t s) {
 void sparse_transpose_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
                           fr_t* t_vals, hipStream_t s) {
   if (transpose_trivial(A, t_row_starts, s)) return;
-  Scratch keys(4 * A.nnz, s), perm(4 * A.nnz, s);
-  sort_by_column(A, nullptr, SinkOut{keys.as<uint32_t>(), t_cols, perm.as<uint32_t>()}, s);
-  k_gather_vals<<<nblocks(A.nnz, 256), 256, 0, s>>>(perm.as<uint32_t>(), A.vals, A.nnz, t_vals);
+  const Plan pl = make_plan(A);
+  const size_t sb = sort_bytes(pl);
+  Scratch ws(sb + Carve::size({4 * A.nnz, 4 * A.nnz}), s);
+  Carve c{ws.as<char>(), sb};
+  uint32_t* keys = c.take<uint32_t>(A.nnz);
+  uint32_t* perm = c.take<uint32_t>(A.nnz);
+  sort_by_column(A, pl, nullptr, SinkOut{keys, t_cols, perm}, Carve{ws.as<char>()}, s);
+  k_gather_vals<<<nblocks(A.nnz + 1, 256), 256, 0, s>>>(perm, keys, A.vals, A.nnz, A.ncols, t_vals,
+                                                          t_row_starts);
   PK_HIP(hipGetLastError());
-  fill_row_starts(keys.as<uint32_t>(), A.nnz, A.ncols, t_row_starts, s);
 }
 
 void sparse_transpose_interned_dev(const CsrDev& A, uint32_t* t_row_starts, uint32_t* t_cols,
                                    uint32_t* t_ids, hipStream_t s) {
   if (transpose_trivial(A, t_row_starts, s)) return;
-  Scratch keys(4 * A.nnz, s);
-  sort_by_column(A, A.ids, SinkOut{keys.as<uint32_t>(), t_cols, t_ids}, s);
-  fill_row_starts(keys.as<uint32_t>(), A.nnz, A.ncols, t_row_starts, s);
+  const Plan pl = make_plan(A);
+  const size_t sb = sort_bytes(pl);
+  Scratch ws(sb + Carve::size({4 * A.nnz}), s);
+  Carve c{ws.as<char>(), sb};
+  uint32_t* keys = c.take<uint32_t>(A.nnz);
+  sort_by_column(A, pl, A.ids, SinkOut{keys, t_cols, t_ids}, Carve{ws.as<char>()}, s);
+  fill_row_starts(keys, A.nnz, A.ncols, t_row_starts, s);
 }
 
 void transposed_eq_mul_dev(const CsrDev& At, const fr_t* alpha, int m, fr_t* out,
