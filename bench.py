@@ -645,7 +645,17 @@ def workload_sumcheck(pk, hip, s, args):
             pk._call("pkgpu_sparse_transpose_device", n, n, rs.ptr, k, cs.ptr, vs.ptr,
                      trs.ptr, tcs.ptr, tvs.ptr, s)
 
+    deq = pk.DeviceBuffer(32 * n)
+
     def at_eq_all():
+        # multiply_transposed_by_eq_alpha (sumcheck.rs:203-227): one eq table
+        # truncated to the constraint count, then the three products
+        pk._call("pkgpu_eq_evaluations_device", dalpha.ptr, m, n, deq.ptr, s)
+        for (trs, tcs, tvs), k, out in zip(tmats, nnz, outs):
+            pk._call("pkgpu_sparse_mul_device", n, n, trs.ptr, k, tcs.ptr, tvs.ptr, deq.ptr,
+                     out.ptr, s)
+
+    def at_eq_per_matrix():
         for (trs, tcs, tvs), k, out in zip(tmats, nnz, outs):
             pk._call("pkgpu_multiply_transposed_by_eq_alpha_device", n, n, trs.ptr, k, tcs.ptr,
                      tvs.ptr, dalpha.ptr, m, out.ptr, s)
@@ -665,6 +675,19 @@ def workload_sumcheck(pk, hip, s, args):
         t_tr.append(hip.elapsed_ms(e0, e1))
         t_eq.append(hip.elapsed_ms(e1, e2))
     tr_ms, eq_ms = statistics.mean(t_tr), statistics.mean(t_eq)
+    at_eq_per_matrix()
+    hip.sync(s)
+    t_pm = []
+    for _ in range(3):
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, s)
+        at_eq_per_matrix()
+        hip.record(e1, s)
+        hip.sync(s)
+        t_pm.append(hip.elapsed_ms(e0, e1))
+    pm_ms = statistics.mean(t_pm)
+    at_eq_all()  # the checked outputs come from the shared-table path
+    hip.sync(s)
     # parity of the last products against the oracle (transpose + eq + SpMV)
     eq_h = pyoracle.eq_evals(res["alphas"], n)
     at_ok = all(np.array_equal(out.download(shape=(n, 4)),
@@ -720,9 +743,10 @@ def workload_sumcheck(pk, hip, s, args):
                      trs.ptr, tcs.ptr, tis.ptr, s)
 
     def at_eq_interned_all():
+        pk._call("pkgpu_eq_evaluations_device", dalpha.ptr, m, n, deq.ptr, s)
         for (trs, tcs, tis), k, out in zip(tids, nnz, iouts):
-            pk._call("pkgpu_multiply_transposed_by_eq_alpha_interned_device", n, n, trs.ptr, k, tcs.ptr,
-                     tis.ptr, interner.ptr, 16, dalpha.ptr, m, out.ptr, s)
+            pk._call("pkgpu_sparse_mul_interned_device", n, n, trs.ptr, k, tcs.ptr, tis.ptr,
+                     interner.ptr, 16, deq.ptr, out.ptr, s)
 
     transpose_interned_all()
     at_eq_interned_all()
@@ -752,6 +776,7 @@ def workload_sumcheck(pk, hip, s, args):
                       "matches_oracle": ok,
                       "after_sumcheck": {
                           "transpose_x3_ms": tr_ms, "at_eq_alpha_x3_ms": eq_ms,
+                          "at_eq_alpha_x3_ms_eq_per_matrix": pm_ms,
                           "matches_oracle": bool(at_ok),
                           "transpose_hbm_frac": sum(72 * k + 8 * n for k in nnz) / (tr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "at_eq_hbm_frac": (spmv_bytes + 3 * 32 * n) / (eq_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
