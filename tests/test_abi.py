@@ -81,3 +81,41 @@ def test_argument_errors_without_device():
     assert L.pkgpu_rs_next_order(1000, ctypes.byref(out)) == pk.OK and out.value == 1024
     assert L.pkgpu_skyscraper_hash_many(31, None, 0, None, 0) == pk.ERR_SIZE
     assert b"unsupported message size 31" in L.pkgpu_last_error()
+
+
+def test_transpose_and_eq_argument_errors_without_device():
+    # The interned transpose and the interned At * eq(alpha) product check
+    # their arguments on the host before any device work.
+    import numpy as np
+
+    import provekit_amd as pk
+    L = pk.lib()
+    u32p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rs, cols, ids = (np.zeros(1, np.uint32), np.array([2], np.uint32), np.zeros(1, np.uint32))
+    trs, tcs, tis = np.zeros(2, np.uint32), np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+    # column index beyond num_cols (host form validates like SparseMatrix)
+    assert L.pkgpu_sparse_transpose_interned(1, 2, u32p(rs), 1, u32p(cols), u32p(ids),
+                                             u32p(trs), u32p(tcs), u32p(tis)) == pk.ERR_ARG
+    assert b"column index out of range" in L.pkgpu_last_error()
+    # decreasing row starts
+    rs2, cols2 = np.array([1, 0], np.uint32), np.array([0, 1], np.uint32)
+    ids2 = np.zeros(2, np.uint32)
+    assert L.pkgpu_sparse_transpose_interned(2, 2, u32p(rs2), 2, u32p(cols2), u32p(ids2),
+                                             u32p(trs), u32p(np.zeros(2, np.uint32)),
+                                             u32p(np.zeros(2, np.uint32))) == pk.ERR_ARG
+    assert b"bad row starts" in L.pkgpu_last_error()
+    # entries without ids, outputs missing
+    assert L.pkgpu_sparse_transpose_interned(1, 3, u32p(rs), 1, u32p(cols), None,
+                                             u32p(trs), u32p(tcs), u32p(tis)) == pk.ERR_ARG
+    assert L.pkgpu_sparse_transpose_interned_device(1, 3, u32p(rs), 1, u32p(cols), u32p(ids),
+                                                    u32p(trs), u32p(tcs), None, None) == pk.ERR_ARG
+    assert b"null output" in L.pkgpu_last_error()
+    # the eq table cannot cover more constraints than 2^m
+    assert L.pkgpu_multiply_transposed_by_eq_alpha_interned_device(
+        3, 5, u32p(np.zeros(3, np.uint32)), 0, None, None, None, 0, None, 2, None, None) == pk.ERR_SIZE
+    assert b"num_entries <= full_size" in L.pkgpu_last_error()
+    # ids without an interner table
+    assert L.pkgpu_multiply_transposed_by_eq_alpha_interned_device(
+        1, 1, u32p(rs), 1, u32p(np.zeros(1, np.uint32)), u32p(ids), None, 4, None, 0,
+        u32p(np.zeros(8, np.uint32)), None) == pk.ERR_ARG
+    assert b"null interner" in L.pkgpu_last_error()
