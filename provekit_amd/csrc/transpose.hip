@@ -101,7 +101,9 @@ __global__ __launch_bounds__(256) void k_row_of(const uint32_t* row_starts, uint
   const uint64_t e = r + 1 < rows ? (row_starts[r + 1] < nnz ? row_starts[r + 1] : nnz) : nnz;
   const uint64_t b = row_starts[r] < e ? row_starts[r] : e;
   const uint32_t b32 = (uint32_t)b;
-  const uint64_t lo_e = (uint32_t)__shfl((int)b32, 0, 64);
+  // the first wave also takes entries before row 0's start (row 0), so every
+  // entry gets a row even when row_starts[0] > 0
+  const uint64_t lo_e = r0 == 0 ? 0 : (uint32_t)__shfl((int)b32, 0, 64);
   const uint64_t hi_e = (uint32_t)__shfl((int)(uint32_t)e, last, 64);
   for (uint64_t p0 = lo_e; p0 < hi_e; p0 += 64) {
     const uint64_t p = p0 + lane;
@@ -397,6 +399,7 @@ void fill_row_starts(const uint32_t* sorted_keys, uint64_t nnz, uint64_t ncols,
 bool transpose_trivial(const CsrDev& A, uint32_t* t_row_starts, hipStream_t s) {
   PK_REQUIRE(A.nnz < (1ull << 32) && A.rows < (1ull << 32) && A.ncols < (1ull << 32), PK_ERR_SIZE,
              "transpose: sizes must fit u32 indices");
+  PK_REQUIRE(A.rows > 0 || A.nnz == 0, PK_ERR_ARG, "transpose: entries without rows");
   if (!A.ncols) return true;
   if (!A.nnz) {
     k_fill_u32<<<nblocks(A.ncols, 256), 256, 0, s>>>(t_row_starts, A.ncols, 0u);

@@ -258,3 +258,37 @@ def test_gpu_multiply_transposed_by_eq_alpha_interned(gpu, oracle):
     t = oracle.sparse_transpose(rows, cols, rs, cs, np.ascontiguousarray(interner[ids]))
     want = oracle.sparse_mul(cols, rows, *t, oracle.eq_evals(alpha, rows))
     assert np.array_equal(out.download(shape=(cols, 4)), want)
+
+
+@pytest.mark.gpu
+def test_gpu_transpose_device_entries_before_first_row_start(gpu, oracle):
+    # The device form trusts the row starts; entries before row_starts[0]
+    # (a malformed CSR) are given row 0, so the output is defined and the
+    # same as for row_starts[0] = 0.
+    rng = np.random.default_rng(9)
+    rows, cols = 3000, 500
+    rs, cs, vs = random_csr(rng, rows, cols, seed=9)
+    nnz = cs.shape[0]
+    bad = rs.copy()
+    bad[0] = int(rs[1]) // 2 if rows > 1 else 0
+    print(f"[transpose] row_starts[0]={bad[0]} rows={rows} nnz={nnz}", flush=True)
+    d = [gpu.DeviceBuffer.from_array(x) for x in (bad, cs, vs)]
+    trs, tcs, tvs = gpu.DeviceBuffer(4 * cols), gpu.DeviceBuffer(4 * nnz), gpu.DeviceBuffer(32 * nnz)
+    gpu._call("pkgpu_sparse_transpose_device", rows, cols, d[0].ptr, nnz, d[1].ptr, d[2].ptr,
+              trs.ptr, tcs.ptr, tvs.ptr, None)
+    want = oracle.sparse_transpose(rows, cols, rs, cs, vs)
+    assert np.array_equal(trs.download(np.uint32), want[0])
+    assert np.array_equal(tcs.download(np.uint32), want[1])
+    assert np.array_equal(tvs.download(shape=(nnz, 4)), want[2])
+
+
+@pytest.mark.gpu
+def test_gpu_transpose_rejects_entries_without_rows(gpu):
+    cs = np.zeros(4, np.uint32)
+    d = gpu.DeviceBuffer.from_array(cs)
+    ids = gpu.DeviceBuffer.from_array(np.zeros(4, np.uint32))
+    trs, tcs, tis = gpu.DeviceBuffer(4 * 2), gpu.DeviceBuffer(16), gpu.DeviceBuffer(16)
+    rc = gpu.lib().pkgpu_sparse_transpose_interned_device(0, 2, None, 4, d.ptr, ids.ptr, trs.ptr,
+                                                          tcs.ptr, tis.ptr, None)
+    assert rc == gpu.ERR_ARG
+    assert b"entries without rows" in gpu.lib().pkgpu_last_error()
