@@ -1003,6 +1003,20 @@ int pkgpu_sumcheck_r1cs_round_device(uint64_t* a, uint64_t* b, uint64_t* c, uint
     sync(s);
   });
 }
+namespace {
+// The calling thread's pinned copy of the sumcheck state.  A copy from
+// pageable memory is staged by the runtime and can hold the host until the
+// stream reaches it, so the launches behind it start late; from pinned memory
+// the copies stay asynchronous.  Each call syncs before returning, so the
+// next call on the thread finds it free.  It is kept until the process ends
+// (freeing it from a thread-exit destructor could run after the runtime).
+pk::ScState* pinned_state() {
+  thread_local pk::ScState* p = nullptr;
+  if (!p) PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(pk::ScState), hipHostMallocDefault));
+  return p;
+}
+}  // namespace
+
 int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
                                 const uint64_t* blinding, uint64_t seed, uint64_t* coeffs,
                                 uint64_t* alphas, uint64_t* r, uint64_t* final_claim,
@@ -1016,7 +1030,7 @@ int pkgpu_sumcheck_prove_device(uint64_t* a, uint64_t* b, uint64_t* c, size_t m,
     REQ(!(blinding && m == 0), pk::PK_ERR_SIZE, "attempt to subtract with overflow");
     hipStream_t s = stream_or_thread(stream);
     const size_t size = m ? (size_t(1) << m) : 2;
-    pk::ScState hst;
+    pk::ScState& hst = *pinned_state();
     pk::sc_state_init_host(hst, (int)m, blinding ? 1 : 0, reinterpret_cast<const fr_t*>(blinding));
     Staged dst(sizeof(pk::ScState), s), deq(sizeof(fr_t) * size, s),
         work(sizeof(fr_t) * (pk::sc_partials_elems() + 3), s),
