@@ -100,4 +100,14 @@ struct Scratch {
 // The calling thread's work stream on the current device (default priority).
 hipStream_t thread_stream();
 
+// Host-to-device copy on stream s from pageable host memory.  Copies of 32 MiB
+// and more go through pinned staging buffers filled by several host threads
+// while the previous chunk is in flight; smaller ones are a plain async copy.
+// src may be reused when the call returns.
+void h2d_from_pageable(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// A 4 KiB pinned host buffer of the calling thread (kept until the process
+// ends), for small device-to-host reads that the thread waits on at once.
+void* pinned_small();
+
 }  // namespace pk

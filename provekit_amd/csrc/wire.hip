@@ -22,9 +22,6 @@
 // canonical integers) and is taken to Montgomery form by one product with
 // R^2.  Malformed input raises a device flag, read once at the end; every
 // kernel stays in bounds whatever the bytes say.
-#include <rocprim/device/device_scan.hpp>
-#include <rocprim/iterator/transform_iterator.hpp>
-
 #include <cstring>
 #include <string>
 
@@ -38,33 +35,206 @@ namespace {
 
 inline unsigned nblocks(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-struct EndsVarint {
-  __device__ __host__ uint32_t operator()(uint8_t c) const { return (c & 0x80) ? 0u : 1u; }
+constexpr int kWT = 256;            // threads per workgroup
+constexpr int kVarintBytes = 32;    // bytes per thread
+constexpr uint64_t kVarintBlock = (uint64_t)kWT * kVarintBytes;
+constexpr int kSegPer = 16;         // scan elements per thread
+constexpr uint64_t kSegBlock = (uint64_t)kWT * kSegPer;
+
+// Exclusive scan over the workgroup (kWT threads) of one value per thread
+// under an associative op with identity id; *total is the fold of all.
+// Every thread must call it.
+template <int NT, class T, class Op>
+__device__ __forceinline__ T block_excl_scan(T v, Op op, T id, T* total) {
+  __shared__ T ws[NT / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  T x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(x, o, 64);
+    if (lane >= o) x = op(y, x);
+  }
+  T ex = __shfl_up(x, 1, 64);
+  if (lane == 0) ex = id;
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  T pre = id, tot = id;
+#pragma unroll
+  for (int i = 0; i < NT / 64; i++) {
+    if (i < w) pre = op(pre, ws[i]);
+    tot = op(tot, ws[i]);
+  }
+  __syncthreads();
+  *total = tot;
+  return op(pre, ex);
+}
+
+struct AddU32 {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
 };
 
-// flat[i], nb[i]: value and byte count of varint i (nb 11: longer than ten
-// bytes, or past 64 bits)
-__global__ __launch_bounds__(256) void k_varints(const uint8_t* b, uint64_t len,
-                                                 const uint32_t* idx, uint64_t* flat,
-                                                 uint8_t* nb) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= len) return;
-  const uint8_t c = b[j];
-  if (c & 0x80) return;
-  uint64_t v = c;
-  int n = 1;
-  bool bad = false;
-  for (uint64_t k = j; k > 0 && (b[k - 1] & 0x80); k--) {
-    if (n == 10) {
-      bad = true;
-      break;
-    }
-    v = (v << 7) | (b[k - 1] & 0x7f);
-    n++;
+// varint ends (bytes with the high bit clear) in one 16-byte group
+__device__ __forceinline__ uint32_t ends16(const uint4& q) {
+  const uint32_t hi = __popc(q.x & 0x80808080u) + __popc(q.y & 0x80808080u) +
+                      __popc(q.z & 0x80808080u) + __popc(q.w & 0x80808080u);
+  return 16u - hi;
+}
+
+// The stream lives in a buffer padded to whole workgroups with 0x80 bytes
+// (continuations, so the padding ends no varint).
+__global__ __launch_bounds__(kWT) void k_varint_count(const uint4* b16, uint32_t* bsum) {
+  const uint64_t c = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * (kVarintBytes / 16);
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < kVarintBytes / 16; k++) n += ends16(b16[c + k]);
+  uint32_t total;
+  (void)block_excl_scan<kWT>(n, AddU32(), 0u, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive prefix of v[0, n) in place under op (identity
+// id), the fold of all in *total when total is set.  Each thread takes a
+// contiguous run of the values.
+template <class T, class Op>
+__global__ __launch_bounds__(1024) void k_scan_one(T* v, uint64_t n, Op op, T id, T* total) {
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t i0 = (uint64_t)threadIdx.x * per < n ? (uint64_t)threadIdx.x * per : n;
+  const uint64_t i1 = i0 + per < n ? i0 + per : n;
+  T a = id;
+  for (uint64_t i = i0; i < i1; i++) a = op(a, v[i]);
+  T tot;
+  T run = block_excl_scan<1024>(a, op, id, &tot);
+  for (uint64_t i = i0; i < i1; i++) {
+    const T c = v[i];
+    v[i] = run;
+    run = op(run, c);
   }
-  if (n == 10 && c > 1) bad = true;  // a 10th byte holds bit 63 only
-  flat[idx[j]] = v;
-  nb[idx[j]] = (uint8_t)(bad ? 11 : n);
+  if (total && threadIdx.x == 0) *total = tot;
+}
+
+// flat[i], nb[i]: value and byte count of varint i (nb 11: longer than ten
+// bytes, or past 64 bits), for the varints ending in this thread's bytes.
+// The bytes are read once into registers and decoded forward; a varint that
+// began before the thread's first byte takes its low bytes from the 16 bytes
+// in front (a run of 16 continuation bytes there is already too long).
+__global__ __launch_bounds__(kWT) void k_varint_decode(const uint8_t* b, uint64_t len,
+                                                       const uint32_t* boff, uint64_t* flat,
+                                                       uint8_t* nb) {
+  const uint64_t j0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kVarintBytes;
+  const uint4* b16 = reinterpret_cast<const uint4*>(b + j0);
+  uint4 q[kVarintBytes / 16];
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < kVarintBytes / 16; k++) {
+    q[k] = b16[k];
+    n += ends16(q[k]);
+  }
+  uint32_t total;
+  uint32_t idx = boff[blockIdx.x] + block_excl_scan<kWT>(n, AddU32(), 0u, &total);
+  if (!n) return;
+  uint64_t acc = 0;
+  int cnt = 0;
+  if (j0) {
+    const uint4 pv = b16[-1];
+    const uint32_t pw[4] = {pv.x, pv.y, pv.z, pv.w};
+    int back = 0;
+    bool run = true;
+#pragma unroll
+    for (int i = 15; i >= 0; i--) {
+      run = run && ((pw[i >> 2] >> (8 * (i & 3))) & 0x80u);
+      back += run ? 1 : 0;
+    }
+    if (back == 16) {
+      cnt = 11;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int k = i - (16 - back);  // position in the varint
+        if (k >= 0 && k < 10)
+          acc |= (uint64_t)((pw[i >> 2] >> (8 * (i & 3))) & 0x7fu) << (7 * k);
+      }
+      cnt = back;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kVarintBytes / 16; k++) {
+    const uint32_t w4[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint32_t c = (w4[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      if (j0 + 16 * k + i >= len) break;
+      if (cnt < 10) acc |= (uint64_t)(c & 0x7fu) << (7 * cnt);
+      cnt++;
+      if (!(c & 0x80u)) {
+        const bool bad = cnt > 10 || (cnt == 10 && c > 1);  // a 10th byte holds bit 63 only
+        flat[idx] = acc;
+        nb[idx] = (uint8_t)(bad ? 11 : cnt);
+        idx++;
+        acc = 0;
+        cnt = 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- headers
+// Up to three matrix headers walked on the device from varint `first`, so
+// the host reads them (and the varint count) in one copy instead of one round
+// trip per field.  err: the first failed check, in the host walk's order.
+enum WireHeadErr : uint32_t {
+  kHeadOk = 0,
+  kHeadEnd,      // unexpected end of input
+  kHeadVarint,   // a header field is not a valid varint
+  kHeadRows,     // num_rows differs from the row-start count
+  kHeadDeltas,   // column delta count differs from value count
+  kHeadNoRows,   // entries without rows
+  kHeadSize,     // sizes beyond u32 indices
+};
+struct WireHeads {
+  uint64_t count;
+  uint64_t nr[3], nc[3], n1[3], n3[3], first[3];
+  uint32_t err, pad;
+};
+
+__global__ void k_wire_heads(const uint64_t* flat, const uint8_t* nb, const uint32_t* count_p,
+                             int nmats, WireHeads* out) {
+  if (threadIdx.x || blockIdx.x) return;
+  const uint64_t count = *count_p;
+  out->count = count;
+  uint32_t e = kHeadOk;
+  uint64_t o = 0;
+  for (int m = 0; m < nmats && !e; m++) {
+    auto at = [&](uint64_t i) -> uint64_t {
+      if (e) return 0;
+      if (i >= count) {
+        e = kHeadEnd;
+        return 0;
+      }
+      if (nb[i] > 10) {
+        e = kHeadVarint;
+        return 0;
+      }
+      return flat[i];
+    };
+    const uint64_t nr = at(o), nc = at(o + 1), n1 = at(o + 2);
+    if (!e && n1 > count) e = kHeadEnd;
+    const uint64_t n2 = e ? 0 : at(o + 3 + n1);
+    if (!e && n2 > count) e = kHeadEnd;
+    const uint64_t n3 = e ? 0 : at(o + 4 + n1 + n2);
+    if (!e && (n3 > count || o + 5 + n1 + n2 + n3 > count)) e = kHeadEnd;
+    if (!e && nr != n1) e = kHeadRows;
+    if (!e && n2 != n3) e = kHeadDeltas;
+    if (!e && !n1 && n3) e = kHeadNoRows;
+    if (!e && (n1 >= (1ull << 32) || n3 >= (1ull << 32))) e = kHeadSize;
+    if (e) break;
+    out->nr[m] = nr;
+    out->nc[m] = nc;
+    out->n1[m] = n1;
+    out->n3[m] = n3;
+    out->first[m] = o;
+    o += 5 + n1 + n2 + n3;
+  }
+  out->err = e;
 }
 
 __device__ __forceinline__ bool u32_ok(uint64_t v, uint8_t n) { return n <= 5 && v <= 0xffffffffull; }
@@ -104,21 +274,50 @@ __global__ __launch_bounds__(256) void k_heads(const uint32_t* row_starts, uint6
 }
 
 // segmented sum: (a, b) -> b if b opens a segment, else a + b (mod 2^32),
-// keeping a's head bit; associative
+// keeping a's head bit; associative, identity 0
 struct SegSum {
-  __device__ __host__ uint64_t operator()(uint64_t a, uint64_t b) const {
+  __device__ uint64_t operator()(uint64_t a, uint64_t b) const {
     if (b & kHead) return b;
     return (a & kHead) | ((a + b) & 0xffffffffull);
   }
 };
 
-__global__ __launch_bounds__(256) void k_cols(const uint64_t* scanned, uint64_t nnz, uint64_t ncols,
-                                              uint32_t* cols, uint32_t* err) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nnz) return;
-  const uint32_t c = (uint32_t)scanned[k];
-  if (c >= ncols) atomicOr(err, 1u);
-  cols[k] = c;
+// per workgroup of kSegBlock entries: the fold of its entries
+__global__ __launch_bounds__(kWT) void k_seg_reduce(const uint64_t* x, uint64_t n, uint64_t* agg) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kSegPer;
+  uint64_t a = 0;
+#pragma unroll
+  for (int k = 0; k < kSegPer; k++)
+    if (i0 + k < n) a = SegSum()(a, x[i0 + k]);
+  uint64_t total;
+  (void)block_excl_scan<kWT>(a, SegSum(), (uint64_t)0, &total);
+  if (threadIdx.x == 0) agg[blockIdx.x] = total;
+}
+
+// the inclusive segmented scan from each workgroup's carry-in, written as
+// columns (u32) checked below ncols
+__global__ __launch_bounds__(kWT) void k_seg_cols(const uint64_t* x, uint64_t n, const uint64_t* carry,
+                                                  uint64_t ncols, uint32_t* cols, uint32_t* err) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kSegPer;
+  uint64_t e[kSegPer];
+  uint64_t a = 0;
+#pragma unroll
+  for (int k = 0; k < kSegPer; k++) {
+    e[k] = i0 + k < n ? x[i0 + k] : 0;
+    a = SegSum()(a, e[k]);
+  }
+  uint64_t total;
+  uint64_t run = SegSum()(carry[blockIdx.x], block_excl_scan<kWT>(a, SegSum(), (uint64_t)0, &total));
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < kSegPer; k++) {
+    if (i0 + k >= n) break;
+    run = SegSum()(run, e[k]);
+    const uint32_t c = (uint32_t)run;
+    bad |= c >= ncols;
+    cols[i0 + k] = c;
+  }
+  if (bad) atomicOr(err, 1u);
 }
 
 __global__ __launch_bounds__(256) void k_hydrate(const uint64_t* flat, const uint8_t* nb, uint64_t nnz,
@@ -214,64 +413,59 @@ void interner_to_device(const InternerBytes& ib, fr_t* out, uint32_t* err, hipSt
 
 // The varint stream of bytes b[0, len) on the device.
 struct FlatStream {
-  Scratch bytes, flat, nb;
-  uint64_t count = 0;
+  Scratch bytes, flat, nb, bsum;
+  uint64_t len, count = 0;
   hipStream_t s;
-  FlatStream(const uint8_t* host, uint64_t len, hipStream_t st)
-      : bytes(len, st), flat(8 * (len ? len : 1), st), nb(len ? len : 1, st), s(st) {
+  static uint64_t padded(uint64_t len) {
+    return len ? (len + kVarintBlock - 1) / kVarintBlock * kVarintBlock : kVarintBlock;
+  }
+  // the varints decoded on stream st (no synchronisation)
+  FlatStream(const uint8_t* host, uint64_t n, hipStream_t st)
+      : bytes(padded(n), st), flat(8 * (n ? n : 1), st), nb(n ? n : 1, st),
+        bsum(4 * (padded(n) / kVarintBlock + 1), st), len(n), s(st) {
     PK_REQUIRE(len < (1ull << 32), PK_ERR_SIZE, "wire format: input above 4 GiB");
     if (!len) return;
-    PK_HIP(hipMemcpyAsync(bytes.p, host, len, hipMemcpyHostToDevice, s));
-    Scratch idx(4 * len, s);
-    rocprim::transform_iterator<const uint8_t*, EndsVarint, uint32_t> ends(bytes.as<uint8_t>(),
-                                                                          EndsVarint());
-    size_t tb = 0;
-    PK_HIP(rocprim::exclusive_scan(nullptr, tb, ends, idx.as<uint32_t>(), 0u, (size_t)len,
-                                   rocprim::plus<uint32_t>(), s));
-    Scratch temp(tb ? tb : 16, s);
-    PK_HIP(rocprim::exclusive_scan(temp.p, tb, ends, idx.as<uint32_t>(), 0u, (size_t)len,
-                                   rocprim::plus<uint32_t>(), s));
-    k_varints<<<nblocks(len, 256), 256, 0, s>>>(bytes.as<uint8_t>(), len, idx.as<uint32_t>(),
-                                                flat.as<uint64_t>(), nb.as<uint8_t>());
+    const uint64_t plen = padded(len), blocks = plen / kVarintBlock;
+    h2d_from_pageable(bytes.p, host, len, s);
+    if (plen > len) PK_HIP(hipMemsetAsync(bytes.as<uint8_t>() + len, 0x80, plen - len, s));
+    k_varint_count<<<(unsigned)blocks, kWT, 0, s>>>(bytes.as<uint4>(), bsum.as<uint32_t>());
+    k_scan_one<<<1, 1024, 0, s>>>(bsum.as<uint32_t>(), blocks, AddU32(), 0u,
+                                  bsum.as<uint32_t>() + blocks);
+    k_varint_decode<<<(unsigned)blocks, kWT, 0, s>>>(bytes.as<uint8_t>(), len, bsum.as<uint32_t>(),
+                                                     flat.as<uint64_t>(), nb.as<uint8_t>());
     PK_HIP(hipGetLastError());
-    uint32_t last_idx = 0;
-    PK_HIP(hipMemcpyAsync(&last_idx, idx.as<uint32_t>() + (len - 1), 4, hipMemcpyDeviceToHost, s));
-    PK_HIP(hipStreamSynchronize(s));
-    count = (uint64_t)last_idx + ((host[len - 1] & 0x80) ? 0 : 1);
   }
-  // usize header field i (checked)
-  uint64_t usize_at(uint64_t i) const {
-    PK_REQUIRE(i < count, PK_ERR_ARG, "wire format: unexpected end of input");
-    uint64_t v = 0;
-    uint8_t n = 0;
-    PK_HIP(hipMemcpyAsync(&v, flat.as<uint64_t>() + i, 8, hipMemcpyDeviceToHost, s));
-    PK_HIP(hipMemcpyAsync(&n, nb.as<uint8_t>() + i, 1, hipMemcpyDeviceToHost, s));
+  // The varint count and the headers of nmats matrices stored one after
+  // another from varint 0 (checked in the order of SparseMatrix's fields).
+  void heads(int nmats, WireCsrSizes* z) {
+    PK_REQUIRE(len, PK_ERR_ARG, "wire format: unexpected end of input");
+    const uint64_t blocks = padded(len) / kVarintBlock;
+    Scratch dh(sizeof(WireHeads), s);
+    k_wire_heads<<<1, 64, 0, s>>>(flat.as<uint64_t>(), nb.as<uint8_t>(),
+                                  bsum.as<uint32_t>() + blocks, nmats, dh.as<WireHeads>());
+    PK_HIP(hipGetLastError());
+    WireHeads* h = static_cast<WireHeads*>(pinned_small());
+    PK_HIP(hipMemcpyAsync(h, dh.p, sizeof(WireHeads), hipMemcpyDeviceToHost, s));
     PK_HIP(hipStreamSynchronize(s));
-    PK_REQUIRE(n <= 10, PK_ERR_ARG, "wire format: bad varint");
-    return v;
+    count = h->count;
+    switch (h->err) {
+      case kHeadOk: break;
+      case kHeadEnd: throw Error(PK_ERR_ARG, "wire format: unexpected end of input");
+      case kHeadVarint: throw Error(PK_ERR_ARG, "wire format: bad varint");
+      case kHeadRows: throw Error(PK_ERR_ARG, "wire format: num_rows differs from the row-start count");
+      case kHeadDeltas:
+        throw Error(PK_ERR_ARG, "wire format: column delta count differs from value count");
+      case kHeadNoRows: throw Error(PK_ERR_ARG, "wire format: entries without rows");
+      default: throw Error(PK_ERR_SIZE, "wire format: sizes must fit u32 indices");
+    }
+    for (int m = 0; m < nmats; m++) {
+      z[m].num_rows = h->n1[m];
+      z[m].num_cols = h->nc[m];
+      z[m].nnz = h->n3[m];
+      z[m].first = h->first[m];
+    }
   }
 };
-
-// Header of the matrix whose first varint is `o`; returns the varint after it.
-uint64_t matrix_header(const FlatStream& fs, uint64_t o, WireCsrSizes& z) {
-  const uint64_t nr = fs.usize_at(o), nc = fs.usize_at(o + 1), n1 = fs.usize_at(o + 2);
-  PK_REQUIRE(n1 <= fs.count, PK_ERR_ARG, "wire format: unexpected end of input");
-  const uint64_t n2 = fs.usize_at(o + 3 + n1);
-  PK_REQUIRE(n2 <= fs.count, PK_ERR_ARG, "wire format: unexpected end of input");
-  const uint64_t n3 = fs.usize_at(o + 4 + n1 + n2);
-  PK_REQUIRE(n3 <= fs.count && o + 5 + n1 + n2 + n3 <= fs.count, PK_ERR_ARG,
-             "wire format: unexpected end of input");
-  PK_REQUIRE(nr == n1, PK_ERR_ARG, "wire format: num_rows differs from the row-start count");
-  PK_REQUIRE(n2 == n3, PK_ERR_ARG, "wire format: column delta count differs from value count");
-  PK_REQUIRE(n1 || !n3, PK_ERR_ARG, "wire format: entries without rows");
-  PK_REQUIRE(n1 < (1ull << 32) && n3 < (1ull << 32), PK_ERR_SIZE,
-             "wire format: sizes must fit u32 indices");
-  z.num_rows = n1;
-  z.num_cols = nc;
-  z.nnz = n3;
-  z.first = o;
-  return o + 5 + n1 + n2 + n3;
-}
 
 void matrix_fill(const FlatStream& fs, const WireCsrSizes& z, const fr_t* interner, uint64_t ni,
                  const WireCsrOut& out, uint32_t* err, hipStream_t s) {
@@ -285,28 +479,26 @@ void matrix_fill(const FlatStream& fs, const WireCsrSizes& z, const fr_t* intern
   }
   if (!n3) return;
   PK_REQUIRE(out.cols && out.vals, PK_ERR_ARG, "wire format: null entries");
-  Scratch packed(8 * n3, s), scanned(8 * n3, s);
+  const uint64_t nblk = (n3 + kSegBlock - 1) / kSegBlock;
+  Scratch packed(8 * n3, s), agg(8 * nblk, s);
   k_deltas<<<nblocks(n3, 256), 256, 0, s>>>(flat + o + 4 + n1, nb + o + 4 + n1, n3,
                                             packed.as<uint64_t>(), err);
   k_heads<<<nblocks(n1, 256), 256, 0, s>>>(out.row_starts, n1, n3, packed.as<uint64_t>());
-  PK_HIP(hipGetLastError());
-  size_t tb = 0;
-  PK_HIP(rocprim::inclusive_scan(nullptr, tb, packed.as<uint64_t>(), scanned.as<uint64_t>(),
-                                 (size_t)n3, SegSum(), s));
-  Scratch temp(tb ? tb : 16, s);
-  PK_HIP(rocprim::inclusive_scan(temp.p, tb, packed.as<uint64_t>(), scanned.as<uint64_t>(),
-                                 (size_t)n3, SegSum(), s));
-  k_cols<<<nblocks(n3, 256), 256, 0, s>>>(scanned.as<uint64_t>(), n3, z.num_cols, out.cols, err);
+  k_seg_reduce<<<(unsigned)nblk, kWT, 0, s>>>(packed.as<uint64_t>(), n3, agg.as<uint64_t>());
+  k_scan_one<<<1, 1024, 0, s>>>(agg.as<uint64_t>(), nblk, SegSum(), (uint64_t)0,
+                                static_cast<uint64_t*>(nullptr));
+  k_seg_cols<<<(unsigned)nblk, kWT, 0, s>>>(packed.as<uint64_t>(), n3, agg.as<uint64_t>(),
+                                             z.num_cols, out.cols, err);
   k_hydrate<<<nblocks(n3, 256), 256, 0, s>>>(flat + o + 5 + n1 + n3, nb + o + 5 + n1 + n3,
                                              n3, interner, ni, out.vals, err);
   PK_HIP(hipGetLastError());
 }
 
 void check_flag(uint32_t* err, hipStream_t s, const char* what) {
-  uint32_t h = 0;
-  PK_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, s));
+  uint32_t* h = static_cast<uint32_t*>(pinned_small());
+  PK_HIP(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, s));
   PK_HIP(hipStreamSynchronize(s));
-  PK_REQUIRE(!h, PK_ERR_ARG, std::string("wire format: ") + what);
+  PK_REQUIRE(!*h, PK_ERR_ARG, std::string("wire format: ") + what);
 }
 
 }  // namespace
@@ -327,7 +519,7 @@ WireCsrSizes sparse_matrix_decode_dev(const uint8_t* blob, size_t len, const fr_
                                       uint64_t ni, const WireCsrOut* out, hipStream_t s) {
   FlatStream fs(blob, len, s);
   WireCsrSizes z;
-  matrix_header(fs, 0, z);
+  fs.heads(1, &z);
   if (out) {
     Scratch err(4, s);
     PK_HIP(hipMemsetAsync(err.p, 0, 4, s));
@@ -345,8 +537,7 @@ R1csSizes r1cs_decode_dev(const uint8_t* blob, size_t len, fr_t* interner_out,
   const InternerBytes ib = read_interner(r);
   z.interner_len = ib.count;
   FlatStream fs(blob + r.pos, len - r.pos, s);
-  uint64_t o = 0;
-  for (int m = 0; m < 3; m++) o = matrix_header(fs, o, z.mats[m]);
+  fs.heads(3, z.mats);
   if (interner_out) {
     Scratch err(4, s);
     PK_HIP(hipMemsetAsync(err.p, 0, 4, s));
