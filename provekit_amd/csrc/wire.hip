@@ -36,9 +36,9 @@ namespace {
 inline unsigned nblocks(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 constexpr int kWT = 256;            // threads per workgroup
-constexpr int kVarintBytes = 32;    // bytes per thread
+constexpr int kVarintBytes = 16;    // bytes per thread
 constexpr uint64_t kVarintBlock = (uint64_t)kWT * kVarintBytes;
-constexpr int kSegPer = 16;         // scan elements per thread
+constexpr int kSegPer = 4;          // scan elements per thread
 constexpr uint64_t kSegBlock = (uint64_t)kWT * kSegPer;
 
 // Exclusive scan over the workgroup (kWT threads) of one value per thread
@@ -100,11 +100,31 @@ __global__ __launch_bounds__(1024) void k_scan_one(T* v, uint64_t n, Op op, T id
   const uint64_t per = (n + 1023) / 1024;
   const uint64_t i0 = (uint64_t)threadIdx.x * per < n ? (uint64_t)threadIdx.x * per : n;
   const uint64_t i1 = i0 + per < n ? i0 + per : n;
+  // loads go out eight at a time (the runs are ~40 values at 2^22 entries)
   T a = id;
-  for (uint64_t i = i0; i < i1; i++) a = op(a, v[i]);
+  uint64_t i = i0;
+  for (; i + 8 <= i1; i += 8) {
+    T w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = v[i + k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) a = op(a, w[k]);
+  }
+  for (; i < i1; i++) a = op(a, v[i]);
   T tot;
   T run = block_excl_scan<1024>(a, op, id, &tot);
-  for (uint64_t i = i0; i < i1; i++) {
+  i = i0;
+  for (; i + 8 <= i1; i += 8) {
+    T w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = v[i + k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      v[i + k] = run;
+      run = op(run, w[k]);
+    }
+  }
+  for (; i < i1; i++) {
     const T c = v[i];
     v[i] = run;
     run = op(run, c);
@@ -113,29 +133,33 @@ __global__ __launch_bounds__(1024) void k_scan_one(T* v, uint64_t n, Op op, T id
 }
 
 // flat[i], nb[i]: value and byte count of varint i (nb 11: longer than ten
-// bytes, or past 64 bits), for the varints ending in this thread's bytes.
-// The bytes are read once into registers and decoded forward; a varint that
-// began before the thread's first byte takes its low bytes from the 16 bytes
-// in front (a run of 16 continuation bytes there is already too long).
+// bytes, or past 64 bits), for the varints ending in this workgroup's bytes.
+// Each thread decodes its 16 bytes forward in registers; a varint that began
+// before them takes its low bytes from the 16 in front (the previous lane's,
+// by a shuffle; a run of 16 continuation bytes there is already too long).
+// The workgroup's varints are numbered consecutively, so they are staged in
+// LDS and written out as one contiguous, coalesced range.
 __global__ __launch_bounds__(kWT) void k_varint_decode(const uint8_t* b, uint64_t len,
                                                        const uint32_t* boff, uint64_t* flat,
                                                        uint8_t* nb) {
+  __shared__ uint64_t s_flat[kVarintBlock];
+  __shared__ uint8_t s_nb[kVarintBlock];
+  const int lane = threadIdx.x & 63;
   const uint64_t j0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kVarintBytes;
   const uint4* b16 = reinterpret_cast<const uint4*>(b + j0);
-  uint4 q[kVarintBytes / 16];
-  uint32_t n = 0;
-#pragma unroll
-  for (int k = 0; k < kVarintBytes / 16; k++) {
-    q[k] = b16[k];
-    n += ends16(q[k]);
-  }
+  const uint4 q = b16[0];
+  const uint32_t n = ends16(q);
   uint32_t total;
-  uint32_t idx = boff[blockIdx.x] + block_excl_scan<kWT>(n, AddU32(), 0u, &total);
-  if (!n) return;
-  uint64_t acc = 0;
-  int cnt = 0;
-  if (j0) {
-    const uint4 pv = b16[-1];
+  uint32_t li = block_excl_scan<kWT>(n, AddU32(), 0u, &total);
+  uint4 pv;
+  pv.x = __shfl(q.x, (lane + 63) & 63, 64);
+  pv.y = __shfl(q.y, (lane + 63) & 63, 64);
+  pv.z = __shfl(q.z, (lane + 63) & 63, 64);
+  pv.w = __shfl(q.w, (lane + 63) & 63, 64);
+  if (lane == 0) pv = j0 ? b16[-1] : make_uint4(0, 0, 0, 0);
+  if (n) {
+    uint64_t acc = 0;
+    int cnt = 0;
     const uint32_t pw[4] = {pv.x, pv.y, pv.z, pv.w};
     int back = 0;
     bool run = true;
@@ -155,25 +179,28 @@ __global__ __launch_bounds__(kWT) void k_varint_decode(const uint8_t* b, uint64_
       }
       cnt = back;
     }
-  }
-#pragma unroll
-  for (int k = 0; k < kVarintBytes / 16; k++) {
-    const uint32_t w4[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const uint32_t c = (w4[i >> 2] >> (8 * (i & 3))) & 0xffu;
-      if (j0 + 16 * k + i >= len) break;
+      if (j0 + i >= len) break;
       if (cnt < 10) acc |= (uint64_t)(c & 0x7fu) << (7 * cnt);
       cnt++;
       if (!(c & 0x80u)) {
         const bool bad = cnt > 10 || (cnt == 10 && c > 1);  // a 10th byte holds bit 63 only
-        flat[idx] = acc;
-        nb[idx] = (uint8_t)(bad ? 11 : cnt);
-        idx++;
+        s_flat[li] = acc;
+        s_nb[li] = (uint8_t)(bad ? 11 : cnt);
+        li++;
         acc = 0;
         cnt = 0;
       }
     }
+  }
+  __syncthreads();
+  const uint64_t base = boff[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < total; i += kWT) {
+    flat[base + i] = s_flat[i];
+    nb[base + i] = s_nb[i];
   }
 }
 
@@ -282,13 +309,30 @@ struct SegSum {
   }
 };
 
+// this thread's kSegPer (= 4) entries, one 32-byte run; past n they are the
+// identity
+__device__ __forceinline__ void seg_load4(const uint64_t* x, uint64_t n, uint64_t i0, uint64_t (&e)[4]) {
+  if (i0 + 4 <= n) {
+    const ulonglong2* v = reinterpret_cast<const ulonglong2*>(x + i0);
+    const ulonglong2 a = v[0], b = v[1];
+    e[0] = a.x;
+    e[1] = a.y;
+    e[2] = b.x;
+    e[3] = b.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) e[k] = i0 + k < n ? x[i0 + k] : 0;
+  }
+}
+
 // per workgroup of kSegBlock entries: the fold of its entries
 __global__ __launch_bounds__(kWT) void k_seg_reduce(const uint64_t* x, uint64_t n, uint64_t* agg) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kSegPer;
+  uint64_t e[kSegPer];
+  seg_load4(x, n, i0, e);
   uint64_t a = 0;
 #pragma unroll
-  for (int k = 0; k < kSegPer; k++)
-    if (i0 + k < n) a = SegSum()(a, x[i0 + k]);
+  for (int k = 0; k < kSegPer; k++) a = SegSum()(a, e[k]);
   uint64_t total;
   (void)block_excl_scan<kWT>(a, SegSum(), (uint64_t)0, &total);
   if (threadIdx.x == 0) agg[blockIdx.x] = total;
@@ -300,22 +344,26 @@ __global__ __launch_bounds__(kWT) void k_seg_cols(const uint64_t* x, uint64_t n,
                                                   uint64_t ncols, uint32_t* cols, uint32_t* err) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kWT + threadIdx.x) * kSegPer;
   uint64_t e[kSegPer];
+  seg_load4(x, n, i0, e);
   uint64_t a = 0;
 #pragma unroll
-  for (int k = 0; k < kSegPer; k++) {
-    e[k] = i0 + k < n ? x[i0 + k] : 0;
-    a = SegSum()(a, e[k]);
-  }
+  for (int k = 0; k < kSegPer; k++) a = SegSum()(a, e[k]);
   uint64_t total;
   uint64_t run = SegSum()(carry[blockIdx.x], block_excl_scan<kWT>(a, SegSum(), (uint64_t)0, &total));
+  uint32_t c[kSegPer];
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < kSegPer; k++) {
-    if (i0 + k >= n) break;
     run = SegSum()(run, e[k]);
-    const uint32_t c = (uint32_t)run;
-    bad |= c >= ncols;
-    cols[i0 + k] = c;
+    c[k] = (uint32_t)run;
+    bad |= i0 + k < n && c[k] >= ncols;
+  }
+  if (i0 + 4 <= n) {
+    *reinterpret_cast<uint4*>(cols + i0) = make_uint4(c[0], c[1], c[2], c[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kSegPer; k++)
+      if (i0 + k < n) cols[i0 + k] = c[k];
   }
   if (bad) atomicOr(err, 1u);
 }
