@@ -54,9 +54,8 @@ struct Staged {
   T* get() const { return buf.as<T>(); }
 };
 
-void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
-  if (bytes) PK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
-}
+// host slices of 32 MiB and more go through the pinned staging pipeline
+void h2d(void* d, const void* h, size_t bytes, hipStream_t s) { pk::h2d_from_pageable(d, h, bytes, s); }
 void d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
   if (bytes) PK_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
 }
