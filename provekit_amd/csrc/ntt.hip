@@ -98,7 +98,7 @@ struct PassShape {
 // x is first brought below 2p, the twiddle product m = w * y is below 2p for
 // y < 4p (w canonical), and x + m, x - m + 2p stay below 4p < 2^256.
 template <int TL, int BPT>
-__device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, fr_t* lds) {
+__device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, fr_t* lds, fr_t* tws) {
   constexpr int T = PassShape<TL, BPT>::T;
   constexpr int N = 1 << TL;
   const int tid = threadIdx.x;
@@ -110,20 +110,31 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   const uint64_t base = h * (Q << a.r) + j0;
   const uint32_t jmask = (1u << jb) - 1;
 
-  // Twiddle of butterfly k of this thread in stage t: rr[(h << t) | (w0 >> (r - t))].
-  // Each stage's twiddles are loaded one stage ahead (the loads stay in
-  // flight across the barrier, which waits on LDS only), so the global load
-  // latency hides behind the previous stage's products instead of opening
-  // every stage.
+  // Twiddle of butterfly k of this thread in stage t: rr[(h << t) | (w0 >> (r - t))],
+  // one of the 2^t entries rr[(h << t) | j], j < 2^t, consecutive in the table.
+  // The early stages (t < tl, tl = r - 2) share them across the tile: they
+  // are staged in LDS once per tile (coalesced loads, tws[2^t - 1 + j],
+  // 2^tl - 1 entries, a quarter of the tile).  In the last two stages every
+  // butterfly has its own, so those come from the table, loaded one stage
+  // ahead (the loads stay in flight across the barrier, which waits on LDS
+  // only).
+  const int tl = a.r > 2 ? a.r - 2 : 0;
   auto tw_index = [&](int t, int k) -> uint64_t {
     const int pos = a.r - 1 - t;
     const int wb = (tid + k * T) >> jb;
     const int w0 = ((wb >> pos) << (pos + 1)) | (wb & ((1 << pos) - 1));
     return (h << t) | ((uint64_t)w0 >> (a.r - t));
   };
-  fr_t tw[BPT];
+  for (int i = tid; i < (1 << tl) - 1; i += T) {
+    const int t = 31 - __builtin_clz((unsigned)(i + 1));
+    const uint64_t j = (uint64_t)(i + 1 - (1 << t));
+    tws[i] = fr_load(a.rr + ((h << t) | j));
+  }
+  fr_t tw_next[BPT];
+  if (tl == 0 && a.r > 0) {
 #pragma unroll
-  for (int k = 0; k < BPT; k++) tw[k] = fr_load(a.rr + tw_index(0, k));
+    for (int k = 0; k < BPT; k++) tw_next[k] = fr_load(a.rr + tw_index(0, k));
+  }
 
   for (int e = tid; e < N; e += T) {
     const uint64_t g = base + (uint64_t)(e >> jb) * Q + (e & jmask);
@@ -140,16 +151,18 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   }
   __syncthreads();
 
-  // stage t with its twiddles in cur; the next stage's go into nxt (two
-  // alternating register sets, so nothing is copied between stages)
-  auto stage = [&](int t, fr_t (&cur)[BPT], fr_t (&nxt)[BPT]) {
+  for (int t = 0; t < a.r; t++) {
     const int pos = a.r - 1 - t;
     const int span = 1 << (pos + jb);
-    fr_t x[BPT], y[BPT];
+    fr_t x[BPT], y[BPT], w[BPT];
     int e0[BPT];
-    if (t + 1 < a.r) {
+    if (t >= tl) {
 #pragma unroll
-      for (int k = 0; k < BPT; k++) nxt[k] = fr_load(a.rr + tw_index(t + 1, k));
+      for (int k = 0; k < BPT; k++) w[k] = tw_next[k];
+    }
+    if (t + 1 < a.r && t + 1 >= tl) {
+#pragma unroll
+      for (int k = 0; k < BPT; k++) tw_next[k] = fr_load(a.rr + tw_index(t + 1, k));
     }
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
@@ -160,21 +173,17 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
       e0[k] = (w0 << jb) | jl;
       x[k] = lds[e0[k]];
       y[k] = lds[e0[k] + span];
+      if (t < tl) w[k] = tws[(1 << t) - 1 + (w0 >> (a.r - t))];
     }
 #pragma unroll
     for (int k = 0; k < BPT; k++) {
-      const fr_t m = fr_mul(cur[k], y[k]);
+      const fr_t m = fr_mul(w[k], y[k]);
       fr_t u = x[k];
       cond_sub<2>(u);
       lds[e0[k]] = fr_add_raw(u, m);
       lds[e0[k] + span] = fr_sub_plus_2p(u, m);
     }
     __syncthreads();
-  };
-  fr_t tw2[BPT];
-  for (int t = 0; t < a.r; t += 2) {
-    stage(t, tw, tw2);
-    if (t + 1 < a.r) stage(t + 1, tw2, tw);
   }
 
   for (int e = tid; e < N; e += T) {
@@ -191,7 +200,8 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
 template <int TL, int BPT>
 __global__ __launch_bounds__((PassShape<TL, BPT>::T)) void k_ntt_pass(PassArgs a) {
   __shared__ fr_t lds[1 << TL];
-  ntt_pass_tile<TL, BPT>(a, blockIdx.x, lds);
+  __shared__ fr_t tws[TL > 2 ? 1 << (TL - 2) : 1];
+  ntt_pass_tile<TL, BPT>(a, blockIdx.x, lds, tws);
 }
 
 // Batched pass: workgroup b runs tile map[b].y of descriptor map[b].x, so one
@@ -200,8 +210,9 @@ template <int TL, int BPT>
 __global__ __launch_bounds__((PassShape<TL, BPT>::T)) void k_ntt_pass_batch(const PassArgs* descs,
                                                                           const uint2* map) {
   __shared__ fr_t lds[1 << TL];
+  __shared__ fr_t tws[TL > 2 ? 1 << (TL - 2) : 1];
   const uint2 m = map[blockIdx.x];
-  ntt_pass_tile<TL, BPT>(descs[m.x], m.y, lds);
+  ntt_pass_tile<TL, BPT>(descs[m.x], m.y, lds, tws);
 }
 
 struct PassVariant {
