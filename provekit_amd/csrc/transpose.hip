@@ -226,8 +226,12 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(Src src, Sink sink, 
     const uint64_t idx = base + (uint64_t)j * 64 + lane;
     const bool valid = idx < n;
     const uint32_t d = (key[j] >> shift) & (bins - 1);
+    // eight ballots whatever the digit width: bits at and above digit_bits
+    // are zero in every lane, so their rounds leave peers unchanged, and the
+    // loop unrolls
     uint64_t peers = __ballot(valid);
-    for (int b = 0; b < digit_bits; b++) {
+#pragma unroll
+    for (int b = 0; b < kMaxDigitBits; b++) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bal = __ballot(bit);
       peers &= bit ? bal : ~bal;
