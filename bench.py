@@ -775,18 +775,21 @@ def workload_sumcheck(pk, hip, s, args):
                      {"stages_ms": {"spmv_x3": spmv_ms, "sumcheck": sc_ms},
                       "matches_oracle": ok,
                       "after_sumcheck": {
-                          "transpose_x3_ms": tr_ms, "at_eq_alpha_x3_ms": eq_ms,
-                          "at_eq_alpha_x3_ms_eq_per_matrix": pm_ms,
-                          "matches_oracle": bool(at_ok),
-                          "transpose_hbm_frac": sum(72 * k + 8 * n for k in nnz) / (tr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                          "at_eq_hbm_frac": (spmv_bytes + 3 * 32 * n) / (eq_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "transpose_x3_ms": itr_ms, "at_eq_alpha_x3_ms": ieq_ms,
+                          "matches_oracle": bool(iat_ok),
+                          "storage": "interned: the reference's SparseMatrix holds u32 value ids, which its "
+                                     "transpose moves; At * eq(alpha) reads them through the 16-entry interner",
+                          "transpose_hbm_frac": sum(24 * k + 8 * n for k in nnz) / (itr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "note": "not in value: SparseMatrix::transpose of A, B, C and At/Bt/Ct * eq(alpha) "
-                                  "(whir_r1cs.rs:172-173) on hydrated 32-byte values; algorithmic bytes "
-                                  "72 nnz + 8 n per transpose",
-                          "interned": {"transpose_x3_ms": itr_ms, "at_eq_alpha_x3_ms": ieq_ms,
-                                       "matches_oracle": iat_ok,
-                                       "note": "the same on the reference's storage: u32 ids moved by the "
-                                               "transpose, At * eq(alpha) through the 16-entry interner"}},
+                                  "(whir_r1cs.rs:172-173, sumcheck.rs:194-227); algorithmic bytes 24 nnz + 8 n per "
+                                  "interned transpose, 72 nnz + 8 n per hydrated one",
+                          "hydrated": {
+                              "transpose_x3_ms": tr_ms, "at_eq_alpha_x3_ms": eq_ms,
+                              "at_eq_alpha_x3_ms_eq_per_matrix": pm_ms,
+                              "matches_oracle": bool(at_ok),
+                              "transpose_hbm_frac": sum(72 * k + 8 * n for k in nnz) / (tr_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                              "at_eq_hbm_frac": (spmv_bytes + 3 * 32 * n) / (eq_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                              "note": "the same with 32-byte values per entry (the transpose gathers them)"}},
                       "roofline_spmv": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms / 1e3) / 1e9,
                                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                         "frac": spmv_bytes / (spmv_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
