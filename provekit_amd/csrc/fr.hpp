@@ -276,8 +276,8 @@ __device__ __forceinline__ fr_t fr_sub_canon(const fr_t& a, const fr_t& b) {
 // the compiler follows every asm statement with a wait state before its
 // outputs are used, which per multiply-add cost a lone wave ~18 % of its
 // product latency (1700 -> 1400 cycles measured; the same rate at full
-// occupancy).  The quotient digit -(x + (x << 28)) is asm too: written in C
-// the compiler folds it back into a v_mul_lo_u32.
+// occupancy).  The quotient digit x * 0xefffffff mod 2^32 is asm too (the low
+// word of a v_mad_u64_u32): written in C the compiler emits v_mul_lo_u32.
 // Operand bound: a_7 + b_7 <= 2^32 - 18 on the top limbs, which lazy x lazy
 // (0.76 * 2^32) and canonical x any value below 4p (0.95 * 2^32) meet.  In
 // columns 7-14 the products that hold a top limb open the column and are
@@ -288,7 +288,7 @@ __device__ __forceinline__ fr_t fr_mul(const fr_t& a, const fr_t& b) {
   fr_t r;
   uint64_t c01 = 0;
   uint32_t c2 = 0;
-  uint64_t cy[3];
+  uint64_t cy[3], mq;
 #include "fr_mul_asm.inc"
   (void)cy;
   return r;
@@ -312,7 +312,7 @@ __device__ __forceinline__ fr_t fr_sqr(const fr_t& a) {
   fr_t r;
   uint64_t c01 = 0;
   uint32_t c2 = 0;
-  uint64_t cy[3];
+  uint64_t cy[3], mq;
 #include "fr_sqr_asm.inc"
   (void)cy;
   return r;

@@ -17,9 +17,11 @@ lower rate.
 
 Algorithm (finely integrated product scanning): column k of a*b and of m*p
 accumulates in the 96-bit (c2 : c01); for k < 8 the quotient digit
-m_k = lo * (-p^-1 mod 2^32) = -(lo + (lo << 28)) makes the low word cancel
-(two full-rate instructions: written in C the compiler folds them back into
-one v_mul_lo_u32), for k >= 8 the low word is output digit k - 8; then the
+m_k = lo * (-p^-1 mod 2^32) = lo * 0xefffffff mod 2^32 makes the low word
+cancel: it is the low word of one v_mad_u64_u32 (the instruction the
+products are measured full-rate with; written as a 32-bit product in C the
+compiler emits v_mul_lo_u32 instead, and -(lo + (lo << 28)) took two
+instructions); for k >= 8 the low word is output digit k - 8; then the
 accumulator shifts down one word (in C: register renaming and one move).
 The first add-with-carry of each column writes c2 from zero instead of
 adding into it, which saves a zeroing move per column (15 per product: on a
@@ -148,13 +150,17 @@ def column(k, square=False):
         out.append(f'  asm("{text}"\n               : {outs}\n'
                    f'               : {", ".join(ops)});')
     if k < 8:
-        # m_k from the column's low word, then m_k * p0 into the column; the
-        # carry add waits two states for the multiply-add's carry
-        out.append(f'  asm("v_lshl_add_u32 %3, %4, 28, %4\\n\\tv_sub_u32 %3, 0, %3\\n\\t'
-                   f'v_mad_u64_u32 %0, %2, %3, %5, %0\\n\\ts_nop 1\\n\\t'
+        # m_k = the low word of lo * 0xefffffff (its carry-out is never set:
+        # the 64-bit product plus 0 cannot pass 2^64), then m_k * p0 into the
+        # column; the carry add waits two states for the multiply-add's carry
+        out.append(f'  asm("v_mad_u64_u32 %0, %1, %2, %3, 0"\n'
+                   f'               : "=v"(mq), "=s"(cy[0])\n'
+                   f'               : "v"((uint32_t)c01), "s"(0xefffffffu));')
+        out.append(f'  m[{k}] = (uint32_t)mq;')
+        out.append(f'  asm("v_mad_u64_u32 %0, %2, %3, %4, %0\\n\\ts_nop 1\\n\\t'
                    f'v_addc_co_u32_e64 %1, %2, 0, %1, %2"\n'
-                   f'               : "+v"(c01), "+v"(c2), "=&s"(cy[0]), "=&v"(m[{k}])\n'
-                   f'               : "v"((uint32_t)c01), "s"(p_limb(0)));')
+                   f'               : "+v"(c01), "+v"(c2), "=&s"(cy[0])\n'
+                   f'               : "v"(m[{k}]), "s"(p_limb(0)));')
     else:
         out.append(f"  r.l[{k - 8}] = (uint32_t)c01;")
     out.append("  c01 = (c01 >> 32) | ((uint64_t)c2 << 32);" if carrying else "  c01 >>= 32;")
@@ -174,11 +180,11 @@ def write(name, head, square):
 def main():
     write("fr_mul_asm.inc", "The body of fr_mul (fr.hpp):\n"
           "// in a, b (lazy, or one canonical and one < 4p: a_7 + b_7 <= 2^32 - 18);\n"
-          "// out r = a * b * 2^-256 (lazy); m[8], c01, c2, cy[3] declared there.",
+          "// out r = a * b * 2^-256 (lazy); m[8], mq, c01, c2, cy[3] declared there.",
           False)
     write("fr_sqr_asm.inc", "The body of fr_sqr (fr.hpp):\n"
           "// in a (< 2p) and its doubled limbs u, d; out r = a * a * 2^-256 (lazy);\n"
-          "// m[8], c01, c2, cy[3] declared there.", True)
+          "// m[8], mq, c01, c2, cy[3] declared there.", True)
 
 
 if __name__ == "__main__":

@@ -140,21 +140,26 @@ __device__ __forceinline__ fr_t sky_compress(const fr_t& l, const fr_t& r) {
 // round sit in a 3.4 KB LDS table, filled once per workgroup.
 //   k = floor(x_7 / (p_7 + 1)) = (x_7 * 0xa948e8c5) >> 61 exactly for every
 //   32-bit x_7 (checked exhaustively on the CPU); one full-rate
-//   v_mad_u64_u32 and a shift.  k p <= x_7 2^224 <= x, and
+//   v_mad_u64_u32, then a shift and a mask give the byte offset 32 k.
+//   k p <= x_7 2^224 <= x, and
 //   x - k p < p + (k + 1) 2^224, so x - k p + rc < 2p for every round
 //   constant (all lie below p - 2^250): the lazy range [0, 2p) is kept.
 // Half-round h (0..17) uses row h: rc[h] (rows 0 and 17 have rc = 0, the
 // input, S-box and output reductions use row 0).  Per round: an 8-limb add,
-// the quotient (2), the LDS address (1) and an 8-limb subtraction: 19 VALU
+// the multiple and its offset (3) and an 8-limb subtraction: 19 VALU
 // instructions against 56 for two reduced additions with the constant moved
 // into registers.
+// The same LDS block holds the byte S-box as a 256-entry table (sb), which
+// the table-form bar reads instead of evaluating the bit-sliced form.
 constexpr int kSkyTabRows = 18, kSkyTabMults = 6;
 struct SkyTab {
   uint4 e[kSkyTabRows * kSkyTabMults * 2];  // entry (h, k): e[2 (6h + k)], e[2 (6h + k) + 1]
+  uint8_t sb[256];                          // sb[x] = S-box of byte x
 };
 
 // Fill the table; every thread of the workgroup calls it, then a barrier.
 __device__ __forceinline__ void sky_tab_fill(SkyTab& tab) {
+  for (int t = threadIdx.x; t < 256; t += blockDim.x) tab.sb[t] = (uint8_t)sbox4((uint32_t)t);
   for (int t = threadIdx.x; t < kSkyTabRows * kSkyTabMults; t += blockDim.x) {
     const int h = t / kSkyTabMults, k = t % kSkyTabMults;
     uint32_t v[8];
@@ -173,19 +178,21 @@ __device__ __forceinline__ void sky_tab_fill(SkyTab& tab) {
   }
 }
 
-// floor(x7 / (p_7 + 1)), exact for all 32-bit x7
-__device__ __forceinline__ uint32_t sky_top_mult(uint32_t x7) {
+// 32 floor(x7 / (p_7 + 1)), the byte offset of multiple k within a table
+// row, exact for all 32-bit x7: ((t >> 61) << 5) = (t_hi >> 24) & 0xe0, a
+// shift and a mask after the multiply
+__device__ __forceinline__ uint32_t sky_top_off(uint32_t x7) {
   uint64_t t, cy;
   asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(t), "=s"(cy) : "v"(x7), "s"(0xa948e8c5u));
   (void)cy;
-  return (uint32_t)(t >> 61);
+  return ((uint32_t)(t >> 32) >> 24) & 0xe0u;
 }
 
 // x (any value below 2^256) -> x - (k p - rc[h]) in [0, 2p) (rc = 0: [0, p + 6 2^224))
 __device__ __forceinline__ fr_t sky_tab_reduce(const fr_t& x, const SkyTab& tab, int h) {
-  const uint32_t k = sky_top_mult(x.l[7]);
-  const uint4 a = tab.e[2 * (kSkyTabMults * h) + 2 * k];
-  const uint4 b = tab.e[2 * (kSkyTabMults * h) + 2 * k + 1];
+  const char* row = reinterpret_cast<const char*>(&tab.e[2 * kSkyTabMults * h]) + sky_top_off(x.l[7]);
+  const uint4 a = *reinterpret_cast<const uint4*>(row);
+  const uint4 b = *reinterpret_cast<const uint4*>(row + 16);
   fr_t r;
   uint32_t br = 0;
   r.l[0] = subb(x.l[0], a.x, br);
@@ -199,13 +206,43 @@ __device__ __forceinline__ fr_t sky_tab_reduce(const fr_t& x, const SkyTab& tab,
   return r;
 }
 
+// S-box of the four bytes of a word by table: four byte reads (the byte
+// extracts are the addresses) and two byte permutes plus an OR to merge them,
+// 7 VALU instructions against 16 for sbox4 (shifts, bit-field inserts and
+// 3-input bit operations).  The 256-byte table is 64 dwords, so a wave's
+// reads touch at most two dwords per bank.
+__device__ __forceinline__ uint32_t sbox4_t(uint32_t v, const uint8_t* sb) {
+  const uint32_t b0 = sb[v & 0xffu], b1 = sb[(v >> 8) & 0xffu];
+  const uint32_t b2 = sb[(v >> 16) & 0xffu], b3 = sb[v >> 24];
+  // v_perm_b32 selector bytes: 0-3 pick bytes of the second operand, 4-7 of
+  // the first, 0x0c gives zero
+  const uint32_t lo = __builtin_amdgcn_perm(b1, b0, 0x0c0c0400u);  // [b0, b1, 0, 0]
+  const uint32_t hi = __builtin_amdgcn_perm(b3, b2, 0x04000c0cu);  // [0, 0, b2, b3]
+  return lo | hi;
+}
+
+// lazy -> canonical (fr_canon), with p read from the table (row 0, multiple
+// 1: p - rc[0] = p) instead of moved into registers limb by limb: a VOP2
+// subtract-with-borrow cannot take a literal next to vcc, so fr_canon spends
+// seven moves on the constant
+__device__ __forceinline__ void sky_canon_t(fr_t& x, const SkyTab& tab) {
+  const uint4 a = tab.e[2], b = tab.e[3];
+  const uint32_t pl[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  fr_t d;
+  uint32_t br = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) d.l[i] = subb(x.l[i], pl[i], br);
+#pragma unroll
+  for (int i = 0; i < 8; i++) x.l[i] = br ? x.l[i] : d.l[i];
+}
+
 // bar with the table: the raw S-box output (any 256-bit value) comes back
 // below p + 6 2^224
 __device__ __forceinline__ fr_t sky_bar_t(fr_t x, const SkyTab& tab) {
-  fr_canon(x);
+  sky_canon_t(x, tab);
   fr_t y;
 #pragma unroll
-  for (int i = 0; i < 8; i++) y.l[i] = sbox4(x.l[(i + 4) & 7]);
+  for (int i = 0; i < 8; i++) y.l[i] = sbox4_t(x.l[(i + 4) & 7], tab.sb);
   return sky_tab_reduce(y, tab, 0);
 }
 
@@ -227,7 +264,7 @@ __device__ __forceinline__ fr_t sky_compress_t(fr_t l, fr_t r, const SkyTab& tab
   // l + t < 2p + p + 6 2^224: the row-0 reduction leaves it below
   // p + 6 2^224, one conditional subtraction of p makes it canonical
   fr_t o = sky_tab_reduce(fr_add_raw(l, t), tab, 0);
-  fr_canon(o);
+  sky_canon_t(o, tab);
   return o;
 }
 

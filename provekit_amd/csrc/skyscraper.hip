@@ -2,12 +2,12 @@
 // left fold (SkyscraperHashEngine::hash_many,
 // provekit/common/src/skyscraper/whir.rs:42-100) and binary Merkle trees.
 //
-// Every compression is ~5.0k VALU instructions (14 Montgomery squarings, 4
-// byte-S-box layers, lazy reductions), so these kernels are VALU-bound; one
-// lane owns one message, which keeps every lane busy with no cross-lane
-// traffic.  The upper tree levels are latency-bound (a compression is a long
-// dependent chain on one wave), so they run one wave per workgroup and the
-// last six levels run inside a single wave.
+// Every compression is ~3.9k VALU instructions (14 Montgomery squarings, 4
+// byte-S-box layers read from an LDS table, lazy reductions), so these
+// kernels are VALU-bound; one lane owns one message, which keeps every lane
+// busy with no cross-lane traffic.  The upper tree levels are latency-bound
+// (a compression is a long dependent chain on one wave), so they run one wave
+// per workgroup and the last six levels run inside a single wave.
 #include "kernels.hpp"
 #include "sky.hpp"
 
