@@ -75,6 +75,19 @@ int pkgpu_ntt_nr_device(uint64_t* values, size_t len, size_t codeword_size, size
  * natural order. */
 int pkgpu_intt_rn(uint64_t* values, size_t len);
 int pkgpu_intt_rn_device(uint64_t* values, size_t len, void* stream);
+/* ntt's intt_nr(values) (ntt/src/ntt.rs:333-348, a private fn there that
+ * intt_rn and the crate's tests call): inverse, natural-order evaluations ->
+ * coefficients in bit-reversed order.  PKGPU_ERR_NOT_POW2 where ntt_nr's
+ * assert panics (a length that is not a power of two). */
+int pkgpu_intt_nr(uint64_t* values, size_t len);
+int pkgpu_intt_nr_device(uint64_t* values, size_t len, void* stream);
+/* ntt's reverse_order(values, codeword_size) (ntt/src/ntt.rs:310-323, private
+ * there): the first codeword_size entries of the len-element slice in
+ * bit-reversed order, in place.  PKGPU_ERR_NOT_POW2 where its assert panics
+ * (codeword_size 0 or not a power of two), PKGPU_ERR_SIZE where its swap
+ * would index past the slice (codeword_size > len). */
+int pkgpu_reverse_order(uint64_t* values, size_t len, size_t codeword_size);
+int pkgpu_reverse_order_device(uint64_t* values, size_t len, size_t codeword_size, void* stream);
 
 /* NTT schedule tuning (no counterpart in the reference, whose ntt crate runs
  * one fixed schedule, ntt/src/ntt.rs:150-248).  Every transform above picks

@@ -148,7 +148,8 @@ void or_ntt_nr(ofr* values, size_t len, size_t codeword_size, size_t num_groups)
   interleaved_ntt_nr(rr, values, len, codeword_size, num_groups);
 }
 
-/* reverse_order (ntt/src/ntt.rs:310-323) */
+/* reverse_order (ntt/src/ntt.rs:310-323): the first n entries (n = the
+ * codeword size, a power of two) in bit-reversed order. */
 void or_reverse_order(ofr* values, size_t n) {
   if (n <= 1) return;
   unsigned bits = ilog2(n);
@@ -162,8 +163,9 @@ void or_reverse_order(ofr* values, size_t n) {
   }
 }
 
-/* intt_nr (ntt/src/ntt.rs:333-348) */
-static void intt_nr(ofr* values, size_t n) {
+/* intt_nr (ntt/src/ntt.rs:333-348): natural-order evaluations ->
+ * coefficients in bit-reversed order. */
+void or_intt_nr(ofr* values, size_t n) {
   if (n == 0) return;
   for (size_t i = 1, j = n - 1; i < j; i++, j--) {
     ofr t = values[i];
@@ -178,7 +180,7 @@ static void intt_nr(ofr* values, size_t n) {
 /* intt_rn (ntt/src/ntt.rs:326-330) */
 void or_intt_rn(ofr* values, size_t len) {
   or_reverse_order(values, len);
-  intt_nr(values, len);
+  or_intt_nr(values, len);
   or_reverse_order(values, len);
 }
 

@@ -26,6 +26,7 @@ ofr or_root_of_unity(size_t n);
 void or_roots_table(size_t order, ofr* out); /* order/2 entries */
 void or_ntt_nr(ofr* values, size_t len, size_t codeword_size, size_t num_groups);
 void or_intt_rn(ofr* values, size_t len);
+void or_intt_nr(ofr* values, size_t len);
 void or_reverse_order(ofr* values, size_t n);
 /* RSFr (provekit/common/src/ntt.rs) */
 int or_rs_next_order(size_t size, size_t* order);

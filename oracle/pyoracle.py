@@ -41,6 +41,7 @@ def lib():
             "or_roots_table": ([_sz, _vp], None),
             "or_ntt_nr": ([_vp, _sz, _sz, _sz], None),
             "or_intt_rn": ([_vp, _sz], None),
+            "or_intt_nr": ([_vp, _sz], None),
             "or_reverse_order": ([_vp, _sz], None),
             "or_rs_next_order": ([_sz, ctypes.POINTER(_sz)], _i),
             "or_rs_evaluation_points": ([_sz, _vp, _sz, _vp], None),
@@ -151,8 +152,14 @@ def intt_rn(values: np.ndarray) -> None:
     lib().or_intt_rn(_p(values), values.shape[0])
 
 
-def reverse_order(values: np.ndarray) -> None:
-    lib().or_reverse_order(_p(values), values.shape[0])
+def intt_nr(values: np.ndarray) -> None:
+    lib().or_intt_nr(_p(values), values.shape[0])
+
+
+def reverse_order(values: np.ndarray, codeword_size: Optional[int] = None) -> None:
+    n = values.shape[0] if codeword_size is None else codeword_size
+    assert n <= values.shape[0]
+    lib().or_reverse_order(_p(values), n)
 
 
 def rs_next_order(size: int) -> Optional[int]:

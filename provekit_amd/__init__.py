@@ -55,6 +55,10 @@ SIGNATURES = {
     "pkgpu_ntt_nr_device": [_vp, _sz, _sz, _sz, _vp],
     "pkgpu_intt_rn": [_vp, _sz],
     "pkgpu_intt_rn_device": [_vp, _sz, _vp],
+    "pkgpu_intt_nr": [_vp, _sz],
+    "pkgpu_intt_nr_device": [_vp, _sz, _vp],
+    "pkgpu_reverse_order": [_vp, _sz, _sz],
+    "pkgpu_reverse_order_device": [_vp, _sz, _sz, _vp],
     "pkgpu_rs_next_order": [_sz, ctypes.POINTER(_sz)],
     "pkgpu_rs_generator": [_sz, _vp],
     "pkgpu_rs_evaluation_points": [_sz, _sz, _vp, _sz, _vp],
@@ -224,6 +228,25 @@ def intt_rn(values: np.ndarray) -> None:
     if values.dtype != np.uint64 or not values.flags.c_contiguous:
         raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
     _call("pkgpu_intt_rn", _ptr(values), values.size // 4)
+
+
+def intt_nr(values: np.ndarray) -> None:
+    """ntt's intt_nr (ntt/src/ntt.rs:333-348): in place, natural-order
+    evaluations -> coefficients in bit-reversed order."""
+    _ensure_init()
+    if values.dtype != np.uint64 or not values.flags.c_contiguous:
+        raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
+    _call("pkgpu_intt_nr", _ptr(values), values.size // 4)
+
+
+def reverse_order(values: np.ndarray, codeword_size: Optional[int] = None) -> None:
+    """ntt's reverse_order (ntt/src/ntt.rs:310-323): the first codeword_size
+    elements (all by default) in bit-reversed order, in place."""
+    _ensure_init()
+    if values.dtype != np.uint64 or not values.flags.c_contiguous:
+        raise ValueError("values must be a C-contiguous uint64 (n, 4) array")
+    n = values.size // 4
+    _call("pkgpu_reverse_order", _ptr(values), n, n if codeword_size is None else codeword_size)
 
 
 # ----------------------------------------------------------------- RSFr
@@ -807,6 +830,18 @@ def commit_device(coeffs: DeviceBuffer, n: int, blowup: int, coset: bool, leaf_e
 def ntt_nr_device(values: DeviceBuffer, length: int, codeword_size: int, num_groups: int,
                   stream=None) -> None:
     _call("pkgpu_ntt_nr_device", values.ptr, length, codeword_size, num_groups, stream)
+
+
+def intt_rn_device(values: DeviceBuffer, length: int, stream=None) -> None:
+    _call("pkgpu_intt_rn_device", values.ptr, length, stream)
+
+
+def intt_nr_device(values: DeviceBuffer, length: int, stream=None) -> None:
+    _call("pkgpu_intt_nr_device", values.ptr, length, stream)
+
+
+def reverse_order_device(values: DeviceBuffer, length: int, codeword_size: int, stream=None) -> None:
+    _call("pkgpu_reverse_order_device", values.ptr, length, codeword_size, stream)
 
 
 def hash_many_device(size: int, inp: DeviceBuffer, count: int, out: DeviceBuffer,

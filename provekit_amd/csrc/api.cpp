@@ -97,6 +97,33 @@ void intt_rn(Fr* values, size_t len) {
   sync(s);
 }
 
+// ntt's intt_nr (ntt/src/ntt.rs:333-348), a private helper of intt_rn there.
+void intt_nr(Fr* values, size_t len) {
+  if (len <= 1) return;
+  REQ(pk::is_pow2(len), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  hipStream_t s = pk::thread_stream();
+  pk::Device& d = pk::device_current();
+  Staged b(sizeof(fr_t) * len, s);
+  h2d(b.get<void>(), values, sizeof(fr_t) * len, s);
+  pk::intt_nr_dev(d, b.get<fr_t>(), len, s);
+  d2h(values, b.get<void>(), sizeof(fr_t) * len, s);
+  sync(s);
+}
+
+// ntt's reverse_order (ntt/src/ntt.rs:310-323) on the first codeword_size
+// entries.
+void reverse_order(Fr* values, size_t len, size_t codeword_size) {
+  REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  REQ(codeword_size <= len, pk::PK_ERR_SIZE, "reverse_order: codeword_size above the slice length");
+  if (codeword_size <= 1) return;
+  hipStream_t s = pk::thread_stream();
+  Staged b(sizeof(fr_t) * codeword_size, s);
+  h2d(b.get<void>(), values, sizeof(fr_t) * codeword_size, s);
+  pk::reverse_order_dev(b.get<fr_t>(), codeword_size, s);
+  d2h(values, b.get<void>(), sizeof(fr_t) * codeword_size, s);
+  sync(s);
+}
+
 std::optional<size_t> RSFr::next_order(size_t size) const {
   size_t o = 1;
   while (o < size) o <<= 1;
@@ -381,6 +408,38 @@ int pkgpu_intt_rn_device(uint64_t* values, size_t len, void* stream) {
     REQ(values, pk::PK_ERR_ARG, "intt_rn: null values");
     pk::intt_rn_dev(pk::device_current(), reinterpret_cast<fr_t*>(values), len,
                     stream_or_thread(stream));
+  });
+}
+
+int pkgpu_intt_nr(uint64_t* values, size_t len) {
+  return guarded([&] {
+    REQ(len == 0 || values, pk::PK_ERR_ARG, "intt_nr: null values");
+    intt_nr(reinterpret_cast<Fr*>(values), len);
+  });
+}
+int pkgpu_intt_nr_device(uint64_t* values, size_t len, void* stream) {
+  return guarded([&] {
+    if (len <= 1) return;
+    REQ(values, pk::PK_ERR_ARG, "intt_nr: null values");
+    pk::intt_nr_dev(pk::device_current(), reinterpret_cast<fr_t*>(values), len,
+                    stream_or_thread(stream));
+  });
+}
+int pkgpu_reverse_order(uint64_t* values, size_t len, size_t codeword_size) {
+  return guarded([&] {
+    REQ(codeword_size == 0 || values, pk::PK_ERR_ARG, "reverse_order: null values");
+    reverse_order(reinterpret_cast<Fr*>(values), len, codeword_size);
+  });
+}
+int pkgpu_reverse_order_device(uint64_t* values, size_t len, size_t codeword_size,
+                               void* stream) {
+  return guarded([&] {
+    REQ(pk::is_pow2(codeword_size), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+    REQ(codeword_size <= len, pk::PK_ERR_SIZE, "reverse_order: codeword_size above the slice length");
+    if (codeword_size <= 1) return;
+    REQ(values, pk::PK_ERR_ARG, "reverse_order: null values");
+    pk::reverse_order_dev(reinterpret_cast<fr_t*>(values), codeword_size,
+                          stream_or_thread(stream));
   });
 }
 

@@ -55,8 +55,14 @@ void ntt_nr_dev(Device& d, fr_t* data, uint64_t len, uint64_t codeword, uint64_t
                 hipStream_t s, const fr_t* src = nullptr, uint64_t chunk = 0);
 // ntt::intt_rn on device (uses one scratch buffer of len elements).
 void intt_rn_dev(Device& d, fr_t* data, uint64_t len, hipStream_t s);
+// ntt's intt_nr (natural-order evaluations -> coefficients in bit-reversed
+// order) on device, one scratch buffer of len elements.
+void intt_nr_dev(Device& d, fr_t* data, uint64_t len, hipStream_t s);
 // Bit-reversal permutation of n elements, out-of-place.
 void bitrev_permute_dev(const fr_t* in, fr_t* out, uint64_t n, hipStream_t s);
+// ntt's reverse_order: the first `codeword` (a power of two) entries in
+// bit-reversed order, in place.
+void reverse_order_dev(fr_t* data, uint64_t codeword, hipStream_t s);
 // RSFr::interleaved_encode into `out` (num_messages * codeword elements).
 // msgs: num_messages rows of msg_len (row-major).  coset != 0 evaluates on the
 // coset 5*H instead of H (coefficient of index j scaled by 5^j).

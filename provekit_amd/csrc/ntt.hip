@@ -364,25 +364,55 @@ void ntt_batch_round_dev(const NttPassDesc* descs, const uint2* map, uint32_t bl
 }
 
 namespace {
-// y[i] = x[bitrev((n - i) mod n)]: the reverse_order then values[1..].reverse()
-// of intt_rn/intt_nr (ntt/src/ntt.rs:326-341) as one gather.
-__global__ void k_intt_in(const fr_t* x, fr_t* y, uint64_t n, int bits) {
+// y[i] = x[e((n - i) mod n)] with e = bitrev when `rev` (intt_rn: the
+// reverse_order then values[1..].reverse() of ntt/src/ntt.rs:326-341 as one
+// gather), else the identity (intt_nr: values[1..].reverse() alone).
+__global__ void k_intt_in(const fr_t* x, fr_t* y, uint64_t n, int bits, int rev) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t k = (n - i) & (n - 1);
-  fr_store(y + i, fr_load(x + bitrev(k, bits)));
+  fr_store(y + i, fr_load(x + (rev ? bitrev(k, bits) : k)));
 }
-// out[i] = f * w[bitrev(i)]: the 1/n scaling then reverse_order.
-__global__ void k_intt_out(const fr_t* w, fr_t* out, uint64_t n, int bits, fr_t f) {
+// out[i] = f * w[e(i)]: the 1/n scaling, then reverse_order when `rev`.
+__global__ void k_intt_out(const fr_t* w, fr_t* out, uint64_t n, int bits, fr_t f, int rev) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  fr_t v = fr_mul(f, fr_load(w + bitrev(i, bits)));
+  fr_t v = fr_mul(f, fr_load(w + (rev ? bitrev(i, bits) : i)));
   fr_canon(v);
   fr_store(out + i, v);
 }
 __global__ void k_bitrev(const fr_t* x, fr_t* y, uint64_t n, int bits) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) fr_store(y + i, fr_load(x + bitrev(i, bits)));
+}
+// In place: thread i swaps entries i and bitrev(i) when i < bitrev(i), so
+// every pair has exactly one owner.
+__global__ void k_bitrev_inplace(fr_t* x, uint64_t n, int bits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t r = bitrev(i, bits);
+  if (i < r) {
+    const fr_t a = fr_load(x + i), b = fr_load(x + r);
+    fr_store(x + i, b);
+    fr_store(x + r, a);
+  }
+}
+
+// The inverse transforms share one schedule: gather (k_intt_in), forward
+// NTT out of place in scratch, scale by 1/n and gather back (k_intt_out).
+void intt_dev(Device& d, fr_t* data, uint64_t n, bool bit_reversed, hipStream_t s) {
+  const int bits = log2_exact(n);
+  Scratch tmp(sizeof(fr_t) * n, s);
+  fr_t* w = tmp.as<fr_t>();
+  const unsigned g = (unsigned)((n + 255) / 256);
+  k_intt_in<<<g, 256, 0, s>>>(data, w, n, bits, bit_reversed ? 1 : 0);
+  PK_HIP(hipGetLastError());
+  ntt_nr_dev(d, w, n, n, 1, s);
+  hfr inv_n = hf::inv(hf::from_u64(n));
+  fr_t f;
+  memcpy(&f, &inv_n, 32);
+  k_intt_out<<<g, 256, 0, s>>>(w, data, n, bits, f, bit_reversed ? 1 : 0);
+  PK_HIP(hipGetLastError());
 }
 }  // namespace
 
@@ -392,21 +422,24 @@ void bitrev_permute_dev(const fr_t* in, fr_t* out, uint64_t n, hipStream_t s) {
   PK_HIP(hipGetLastError());
 }
 
+void reverse_order_dev(fr_t* data, uint64_t codeword, hipStream_t s) {
+  PK_REQUIRE(is_pow2(codeword), PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  if (codeword <= 1) return;
+  k_bitrev_inplace<<<(unsigned)((codeword + 255) / 256), 256, 0, s>>>(data, codeword,
+                                                                       log2_exact(codeword));
+  PK_HIP(hipGetLastError());
+}
+
 void intt_rn_dev(Device& d, fr_t* data, uint64_t n, hipStream_t s) {
   if (n <= 1) return;  // 1/1 scaling of a single element is the identity
   PK_REQUIRE(is_pow2(n), PK_ERR_NOT_POW2, "intt_rn: length must be a power of two");
-  const int bits = log2_exact(n);
-  Scratch tmp(sizeof(fr_t) * n, s);
-  fr_t* w = tmp.as<fr_t>();
-  const unsigned g = (unsigned)((n + 255) / 256);
-  k_intt_in<<<g, 256, 0, s>>>(data, w, n, bits);
-  PK_HIP(hipGetLastError());
-  ntt_nr_dev(d, w, n, n, 1, s);
-  hfr inv_n = hf::inv(hf::from_u64(n));
-  fr_t f;
-  memcpy(&f, &inv_n, 32);
-  k_intt_out<<<g, 256, 0, s>>>(w, data, n, bits, f);
-  PK_HIP(hipGetLastError());
+  intt_dev(d, data, n, true, s);
+}
+
+void intt_nr_dev(Device& d, fr_t* data, uint64_t n, hipStream_t s) {
+  if (n <= 1) return;  // 1/1 scaling of a single element is the identity
+  PK_REQUIRE(is_pow2(n), PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
+  intt_dev(d, data, n, false, s);
 }
 
 namespace {

@@ -70,6 +70,72 @@ def test_ntt_errors(gpu):
     gpu.ntt_nr(empty, 0, 1)  # ntt_empty (ntt.rs:507-512)
 
 
+@pytest.mark.parametrize("k", list(range(0, 13)) + [16, 20])
+def test_intt_nr(gpu, oracle, k):
+    # intt_nr (ntt.rs:333-348): natural-order evaluations -> bit-reversed
+    # coefficients; the oracle's restatement is pinned by brute force
+    n = 1 << k
+    v = random_fr(n, seed=300 + k, stream="intt_nr")
+    say(f"intt_nr 2^{k}")
+    a, b = v.copy(), v.copy()
+    gpu.intt_nr(a)
+    oracle.intt_nr(b)
+    assert np.array_equal(a, b)
+    # and it inverts ntt_nr up to the two bit-reversal permutations
+    gpu.reverse_order(a)
+    gpu.ntt_nr(a, n, 1)
+    gpu.reverse_order(a)
+    assert np.array_equal(a, v)
+
+
+@pytest.mark.parametrize("n,codeword", [(1, 1), (2, 2), (64, 16), (40, 32), (1 << 12, 1 << 12),
+                                        (3 << 16, 1 << 17)])
+def test_reverse_order(gpu, oracle, n, codeword):
+    v = random_fr(n, seed=n + codeword, stream="reverse_order")
+    say(f"reverse_order len={n} codeword={codeword}")
+    a, b = v.copy(), v.copy()
+    gpu.reverse_order(a, codeword)
+    oracle.reverse_order(b, codeword)
+    assert np.array_equal(a, b)
+
+
+def test_inverse_device_entry_points(gpu, oracle):
+    n = 1 << 14
+    v = random_fr(n, seed=77, stream="inverse-device")
+    say("intt_nr_device / intt_rn_device / reverse_order_device 2^14")
+    for dev_fn, ref_fn in ((gpu.intt_nr_device, oracle.intt_nr), (gpu.intt_rn_device, oracle.intt_rn)):
+        d = gpu.DeviceBuffer.from_array(v)
+        dev_fn(d, n)
+        gpu.synchronize()
+        ref = v.copy()
+        ref_fn(ref)
+        assert np.array_equal(d.download(shape=(n, 4)), ref)
+    d = gpu.DeviceBuffer.from_array(v)
+    gpu.reverse_order_device(d, n, n // 4)
+    gpu.synchronize()
+    ref = v.copy()
+    oracle.reverse_order(ref, n // 4)
+    assert np.array_equal(d.download(shape=(n, 4)), ref)
+
+
+def test_inverse_errors(gpu):
+    import provekit_amd as pk
+    v = random_fr(12, seed=2)
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        gpu.intt_nr(v)
+    assert e.value.code == pk.ERR_NOT_POW2
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        gpu.reverse_order(v, 0)  # 0.is_power_of_two() is false: the reference asserts
+    assert e.value.code == pk.ERR_NOT_POW2
+    with pytest.raises(pk.ProvekitGpuError) as e:
+        gpu.reverse_order(v, 16)  # past the slice
+    assert e.value.code == pk.ERR_SIZE
+    one = random_fr(1, seed=3)
+    w = one.copy()
+    gpu.intt_nr(w)  # 1/1 scaling: unchanged
+    assert np.array_equal(w, one)
+
+
 @pytest.mark.parametrize("log_msg,log_extra,M,log_mask", [
     (0, 0, 1, 0), (1, 1, 1, 0), (2, 3, 3, 1), (4, 2, 4, 3), (3, 0, 2, 2), (4, 3, 1, 3), (2, 1, 4, 0),
     (10, 2, 1, 0), (12, 1, 2, 4)])

@@ -209,6 +209,53 @@ def test_reverse_order_round_trip(oracle):
     assert np.array_equal(v, w)
 
 
+@pytest.mark.parametrize("n,codeword", [(1, 1), (2, 2), (8, 8), (64, 16), (40, 32), (1024, 1024)])
+def test_reverse_order_definition(oracle, n, codeword):
+    # reverse_order(values, codeword_size) (ntt.rs:310-323): entry i of the
+    # first codeword_size moves to bitrev(i); the rest stay
+    v = np.arange(n * 4, dtype=np.uint64).reshape(n, 4)
+    w = v.copy()
+    oracle.reverse_order(w, codeword)
+    bits = codeword.bit_length() - 1
+    for i in range(codeword):
+        assert np.array_equal(w[bitrev(i, bits)], v[i])
+    assert np.array_equal(w[codeword:], v[codeword:])
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 5, 8])
+def test_intt_nr_brute_force(oracle, k):
+    # intt_nr (ntt.rs:333-348): natural-order evaluations y_i = P(w^i) give
+    # P's coefficients in bit-reversed order, c_{bitrev(j)} at position j
+    N = 1 << k
+    rng = random.Random(500 + k)
+    coeffs = [rng.randrange(P) for _ in range(N)]
+    w = unmont(oracle.root_of_unity(N))
+    evals = []
+    for i in range(N):
+        x, acc = pow(w, i, P), 0
+        for c in reversed(coeffs):
+            acc = (acc * x + c) % P
+        evals.append(acc)
+    v = np.array([mont(e) for e in evals], dtype=np.uint64).reshape(-1, 4)
+    oracle.intt_nr(v)
+    assert [unmont(x) for x in v] == [coeffs[bitrev(j, k)] for j in range(N)]
+
+
+@pytest.mark.parametrize("k", [4, 11])
+def test_intt_nr_inverts_ntt_nr(oracle, k):
+    # ntt_nr gives bit-reversed evaluations; reverse_order, intt_nr and
+    # reverse_order again give the coefficients back
+    n = 1 << k
+    rng = np.random.default_rng(900 + k)
+    v = np.array([mont(int(x)) for x in rng.integers(0, 2**62, n)], dtype=np.uint64).reshape(-1, 4)
+    a = v.copy()
+    oracle.ntt_nr(a, n, 1)
+    oracle.reverse_order(a)
+    oracle.intt_nr(a)
+    oracle.reverse_order(a)
+    assert np.array_equal(a, v)
+
+
 def brute_rs(messages, masks, codeword):
     """RSFr::interleaved_encode closed form: out[i*M + r] = P_r(w^{bitrev(i)})
     where P_r's coefficients are message r then its mask entries
