@@ -856,6 +856,8 @@ def main():
         fn = {"ntt": workload_ntt, "p2merkle": workload_p2merkle, "serving": workload_serving,
               "sumcheck": workload_sumcheck, "pow": workload_pow, "wire": workload_wire}[args.workload]
         print(json.dumps(fn(pk, hip, s, args)), flush=True)
+        if args.copy_probe:
+            copy_probe()
         return
 
     n = 1 << args.log_n
