@@ -280,7 +280,10 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
         const uint32_t di = (uint32_t)descs.size();
         descs.push_back(nplans[i][r]);
         const uint64_t tiles = polys[i].N >> sh.first;
-        for (uint64_t t = 0; t < tiles; t++) map.push_back(make_uint2(di, (uint32_t)t));
+        const NttPassDesc& a = nplans[i][r];
+        const uint64_t T = a.Q >> a.jb;
+        for (uint64_t t = 0; t < tiles; t++)
+          map.push_back(make_uint2(di, (uint32_t)ntt_tile_order(a.src != nullptr, T, tiles / T, t)));
       }
       BatchPlan::Round rd;
       rd.tl = sh.first;

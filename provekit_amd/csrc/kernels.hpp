@@ -26,6 +26,20 @@ struct NttShape {
   int bpt = 0;       // 0: the tile's default
   int min_run = -1;  // -1: 2
 };
+// Tile run by workgroup b of a pass with T tiles per row h and H rows.  A
+// first pass that reads a replicated coset chunk has one row per replica
+// (h = g / chunk), so the H tiles (h, pos) read the same chunk elements.
+// Workgroups go to the 8 XCDs round-robin (b mod 8), each with its own L2:
+// b = 8 (H u + h) + x runs tile (h, 8 u + x), so the H replicas of a
+// position run on one XCD within 8 H workgroups of each other and all but
+// the first read the chunk from that L2.  Other passes, and rows that are
+// not a multiple of 8 tiles, keep b.  A bijection on [0, H T).
+__host__ __device__ inline uint64_t ntt_tile_order(bool replicated, uint64_t T, uint64_t H,
+                                                   uint64_t b) {
+  if (!replicated || (T & 7) != 0 || H == 0) return b;
+  const uint64_t x = b & 7, rest = b >> 3;
+  return (rest % H) * T + (rest / H) * 8 + x;
+}
 bool ntt_shape_supported(int tile_log, int bpt);
 int ntt_default_bpt(int tile_log);
 // The passes of ntt_nr over `data`.

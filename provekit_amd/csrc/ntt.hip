@@ -197,11 +197,16 @@ __device__ __forceinline__ void ntt_pass_tile(const PassArgs& a, uint64_t tile, 
   }
 }
 
+__device__ __forceinline__ uint64_t pass_tile_of(const PassArgs& a, uint64_t b) {
+  const uint64_t T = a.Q >> a.jb;  // tiles per row h
+  return ntt_tile_order(a.src != nullptr, T, gridDim.x / T, b);
+}
+
 template <int TL, int BPT>
 __global__ __launch_bounds__((PassShape<TL, BPT>::T)) void k_ntt_pass(PassArgs a) {
   __shared__ fr_t lds[1 << TL];
   __shared__ fr_t tws[TL > 2 ? 1 << (TL - 2) : 1];
-  ntt_pass_tile<TL, BPT>(a, blockIdx.x, lds, tws);
+  ntt_pass_tile<TL, BPT>(a, pass_tile_of(a, blockIdx.x), lds, tws);
 }
 
 // Batched pass: workgroup b runs tile map[b].y of descriptor map[b].x, so one
