@@ -7,11 +7,16 @@
 //     device work queue (k_hash_queue).
 //
 // The queue holds, in topological order, leaf tasks (64 leaves of one
-// polynomial), node tasks (up to 64 nodes of one tree level) and one tail
-// task per tree (the last levels from <= 64 nodes to the root inside one
+// polynomial), node tasks (64 k nodes of one tree level, k per lane) and one
+// tail task per tree (the last levels from <= 64 nodes to the root inside one
 // wave).  A wave takes the next task index from a global counter, waits for
-// the (at most two) tasks that produce its inputs, runs it and publishes a
-// completion flag.  Tasks are handed out in topological order, so every task a
+// the tasks that produce its inputs (a consecutive range), runs it and
+// publishes a completion flag.  Publishing costs a release fence, which
+// writes back the XCD's L2, and reading costs an acquire fence, which
+// invalidates it: with one node per lane those fences, not the compressions,
+// bound the tree levels (2^23 one-node tasks: 7 ms of the queue's 22.6 ms,
+// against 1.1 ms of compressions).  Wide levels therefore give each lane up
+// to 8 nodes per task.  Tasks are handed out in topological order, so every task a
 // wave waits for was already taken by a running wave that waits only on
 // earlier tasks: the earliest unfinished task always progresses, whatever the
 // residency.  Each wait is bounded; on timeout the wave raises the error flag
@@ -78,18 +83,18 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
     if (id >= ntasks) return;
     const HashTask tk = tasks[id];
     const BatchPolyDev& p = polys[tk.poly];
-    if (!wave_wait(done, tk.dep0, err, kSpinLimit) || !wave_wait(done, tk.dep1, err, kSpinLimit))
-      return;
+    if (!wave_wait_range(done, tk.dep_first, tk.dep_count, err, kSpinLimit)) return;
     // the dependency flags were seen with relaxed loads; one acquire fence
     // (it invalidates this XCD's L2) makes the data they guard visible to the
     // wave.  Leaf tasks read only codewords written by earlier launches and
     // skip it.
-    if (tk.dep0 >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tk.dep_first >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     // The task as steps of one compression each (wave-uniform count):
     //  leaf: acc = m0, then acc = C(acc, m_k), k = 1 .. leaf_elems - 1, the
     //        words read four at a time (one 128-byte line per lane; see
     //        sky_hash_message) through a shifting window;
-    //  node: one step, C(in[2i], in[2i + 1]);
+    //  node: k steps, C(in[2i], in[2i + 1]) for the lane's nodes
+    //        i = 64 k j + 64 step + lane;
     //  tail: level tk.level (<= 64 nodes, one per lane) up to the root, one
     //        step per level, operands from the level below by shuffles.
     const uint32_t kind = tk.kind;
@@ -118,11 +123,11 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
       nsteps = (int)E - 1;
       dst = p.nodes + lane_i;
     } else if (kind == kTaskNode) {
-      const uint64_t nl = p.leaves >> tk.level;
-      act = lane_i < nl;
-      src = p.nodes + level_off(p.leaves, tk.level - 1) + 2 * (act ? lane_i : 0);
-      dst = p.nodes + level_off(p.leaves, tk.level) + lane_i;
-      nsteps = 1;
+      n = p.leaves >> tk.level;  // nodes at this level
+      lane_i = ((uint64_t)tk.j << (6 + tk.k_log)) + t;
+      src = p.nodes + level_off(p.leaves, tk.level - 1);
+      dst = p.nodes + level_off(p.leaves, tk.level);
+      nsteps = 1 << tk.k_log;
     } else {
       n = p.leaves >> tk.level;
       const uint64_t off = level_off(p.leaves, tk.level);
@@ -152,9 +157,11 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
         have--;
         on = act;
       } else if (kind == kTaskNode) {
-        x = fr_load(src);
-        y = fr_load(src + 1);
-        on = act;
+        const uint64_t i = lane_i + (uint64_t)step * kQueueWave;
+        on = i < n;
+        const fr_t* in = src + 2 * (on ? i : 0);
+        x = fr_load(in);
+        y = fr_load(in + 1);
       } else {
         x = fr_shfl(acc, (2 * t) & (kQueueWave - 1));
         y = fr_shfl(acc, (2 * t + 1) & (kQueueWave - 1));
@@ -165,7 +172,7 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
       if (kind == kTaskLeaf) {
         acc = v;
       } else if (kind == kTaskNode) {
-        if (on) fr_store(dst, v);
+        if (on) fr_store(dst + lane_i + (uint64_t)step * kQueueWave, v);
       } else {
         if (on) fr_store(dst + t, v);
         acc = v;
@@ -294,9 +301,12 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
     }
   }
   plan->descs = upload_vec(descs, plan->owned);
-  // hashing queue, topological: all leaf tasks, node tasks level by level, tails
+  // hashing queue, topological: all leaf tasks, node tasks level by level, tails.
+  // prev_first[i] / prev_width[i]: the first task id and the nodes per task
+  // of polynomial i's previous level (its tasks are consecutive).
   std::vector<HashTask> tasks;
-  std::vector<std::vector<int32_t>> prev(specs.size());  // task ids of the previous level
+  std::vector<int32_t> prev_first(specs.size());
+  std::vector<uint64_t> prev_width(specs.size(), kQueueWave), prev_count(specs.size());
   int max_levels = 0;
   std::vector<int> tail_level(specs.size());
   for (size_t i = 0; i < specs.size(); i++) {
@@ -304,29 +314,38 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
     const int lg = log2_exact(n0);
     tail_level[i] = lg > 6 ? lg - 6 : 0;  // first level with <= 64 nodes
     max_levels = std::max(max_levels, tail_level[i]);
-    for (uint64_t j = 0; j * kQueueWave < n0; j++) {
-      prev[i].push_back((int32_t)tasks.size());
-      tasks.push_back(HashTask{(uint32_t)i, 0, (uint32_t)j, kTaskLeaf, -1, -1});
-    }
+    prev_first[i] = (int32_t)tasks.size();
+    for (uint64_t j = 0; j * kQueueWave < n0; j++)
+      tasks.push_back(HashTask{(uint32_t)i, 0, (uint32_t)j, kTaskLeaf, -1, 0, 0, 0});
+    prev_count[i] = tasks.size() - (size_t)prev_first[i];
   }
   for (int l = 1; l <= max_levels; l++) {
     for (size_t i = 0; i < specs.size(); i++) {
       if (l > tail_level[i]) continue;
       const uint64_t nl = polys[i].leaves >> l;
-      std::vector<int32_t> cur;
-      for (uint64_t j = 0; j * kQueueWave < nl; j++) {
-        const int32_t d0 = prev[i][2 * j];
-        const int32_t d1 = 2 * j + 1 < prev[i].size() ? prev[i][2 * j + 1] : -1;
-        cur.push_back((int32_t)tasks.size());
-        tasks.push_back(HashTask{(uint32_t)i, (uint32_t)l, (uint32_t)j, kTaskNode, d0, d1});
+      // nodes per lane: up to 8 while the level keeps >= 32 tasks
+      uint32_t k_log = 0;
+      while (k_log < 3 && (nl >> (6 + k_log + 1)) >= 32) k_log++;
+      const uint64_t width = (uint64_t)kQueueWave << k_log;
+      const int32_t first = (int32_t)tasks.size();
+      for (uint64_t j = 0; j * width < nl; j++) {
+        // inputs: nodes [2 width j, 2 width (j + 1)) of the level below
+        const uint64_t t0 = (2 * width * j) / prev_width[i];
+        const uint64_t t1 = std::min<uint64_t>((2 * width * (j + 1) + prev_width[i] - 1) / prev_width[i],
+                                               prev_count[i]);
+        tasks.push_back(HashTask{(uint32_t)i, (uint32_t)l, (uint32_t)j, kTaskNode,
+                                 prev_first[i] + (int32_t)t0, (uint32_t)(t1 - t0), k_log, 0});
       }
-      prev[i] = cur;
+      prev_first[i] = first;
+      prev_width[i] = width;
+      prev_count[i] = tasks.size() - (size_t)first;
     }
   }
   for (size_t i = 0; i < specs.size(); i++) {
     if (polys[i].leaves <= 1) continue;
-    // level tail_level[i] was produced by exactly one task (<= 64 nodes)
-    tasks.push_back(HashTask{(uint32_t)i, (uint32_t)tail_level[i], 0, kTaskTail, prev[i][0], -1});
+    // level tail_level[i] (<= 64 nodes) was produced by one task
+    tasks.push_back(HashTask{(uint32_t)i, (uint32_t)tail_level[i], 0, kTaskTail, prev_first[i],
+                             (uint32_t)prev_count[i], 0, 0});
   }
   plan->ntasks = (uint32_t)tasks.size();
   plan->tasks = upload_vec(tasks, plan->owned);

@@ -27,9 +27,16 @@ struct BatchPolyDev {
 };
 
 enum : uint32_t { kTaskLeaf = 0, kTaskNode = 1, kTaskTail = 2 };
+// A leaf task hashes leaves [64 j, 64 j + 64); a node task computes nodes
+// [64 k j, 64 k (j + 1)) of one tree level, k = 1 << k_log per lane; a tail
+// task runs the last levels (<= 64 nodes) to the root.  Its inputs come from
+// tasks [dep_first, dep_first + dep_count), consecutive in the queue.
 struct HashTask {
   uint32_t poly, level, j, kind;
-  int32_t dep0, dep1;  // task ids producing the inputs (-1: none)
+  int32_t dep_first;   // -1: no inputs from other tasks
+  uint32_t dep_count;
+  uint32_t k_log;      // node tasks: log2 nodes per lane
+  uint32_t pad;
 };
 
 struct BatchPlan {

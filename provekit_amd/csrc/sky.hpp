@@ -346,23 +346,32 @@ __device__ __forceinline__ uint32_t wave_take(uint32_t* counter) {
   const uint32_t v = __hip_atomic_fetch_add(counter, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return __builtin_amdgcn_readfirstlane(v);
 }
-// Bounded wait for flag `dep` (< 0: none) with relaxed polls, each a single
-// load of one address by the whole wave with a uniform result; gives up when
-// the error flag is raised or after `spin` polls (then raises it).
-__device__ __forceinline__ bool wave_wait(const uint32_t* done, int64_t dep, uint32_t* err,
-                                          uint64_t spin) {
-  if (dep < 0) return true;
-  for (uint64_t it = 0; it < spin; it++) {
-    if (__builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(done + dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-      return true;
-    if (__builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-      return false;
-    __builtin_amdgcn_s_sleep(1);
+// Bounded wait for flags [first, first + count) (first < 0: none): each lane
+// polls one flag of a group of 64 with a relaxed load, and the wave moves on
+// when every flag of the group is set; gives up when the error flag is
+// raised or after `spin` polls (then raises it).
+__device__ __forceinline__ bool wave_wait_range(const uint32_t* done, int64_t first, uint32_t count,
+                                                uint32_t* err, uint64_t spin) {
+  if (first < 0) return true;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t base = 0; base < count; base += 64) {
+    const bool mine = base + lane < count;
+    const uint32_t* f = done + first + base + (mine ? lane : 0);
+    uint64_t it = 0;
+    for (;; it++) {
+      const uint32_t v = mine ? __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1u;
+      if (__all(v != 0)) break;
+      if (it + 1 >= spin) {
+        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+      if (__builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+        return false;
+      __builtin_amdgcn_s_sleep(1);
+    }
   }
-  __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return false;
+  return true;
 }
 __device__ __forceinline__ fr_t fr_shfl(const fr_t& v, int src) {
   fr_t r;
