@@ -45,6 +45,20 @@ def test_batch_uneven_small(gpu, oracle):
     check(gpu, oracle, batch, host, batch.roots())
 
 
+@pytest.mark.parametrize("leaf_elems", [2, 4, 8, 32, 64])
+def test_batch_leaf_sizes(gpu, oracle, leaf_elems):
+    # leaf tasks fold 1..63 words through the four-word window: sizes below,
+    # at and above the window, and trees from 2 leaves up
+    ks = [6, 9, 12, 14, 10]
+    blowups = [2, 4, 4, 2, 2]
+    cosets = [True, False, True, True, False]
+    polys, host = make_batch(gpu, ks, blowups, cosets, seed=500 + leaf_elems)
+    print(f"[case] batch leaf_elems={leaf_elems} k={ks}", flush=True)
+    batch = gpu.BatchCommit(polys, leaf_elems=leaf_elems)
+    batch.run()
+    check(gpu, oracle, batch, host, batch.roots())
+
+
 def test_batch_graph_replays(gpu, oracle):
     ks = [12, 15, 11, 17, 13]
     polys, host = make_batch(gpu, ks, [4, 2, 4, 2, 4], [True] * 5, seed=7)
