@@ -16,6 +16,10 @@
 //    one workgroup, so a whole sumcheck runs without the host.
 // The transcript is the project's Skyscraper transcript (init / absorb1 /
 // round step / squeeze with parallel compressions, defined in sumcheck.hpp).
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "host_fr.hpp"
 #include "kernels.hpp"
 #include "sky.hpp"
@@ -837,17 +841,42 @@ void eq_table_dev(const fr_t* r, int m, uint64_t num_entries, fr_t* out, hipStre
   PK_HIP(hipGetLastError());
 }
 
-static unsigned round_blocks(uint64_t work) {
+// Workgroups of a grid-stride round kernel: one per 256 pairs, at most as
+// many as the device holds at once (the kernel's occupancy times the CUs,
+// at most kMaxRoundBlocks).  Every workgroup strides over the same share of
+// the pairs, so a grid larger than one generation would run its remainder
+// as a second, thinly occupied generation (k_sc_round holds three waves per
+// SIMD: 768 resident workgroups on 256 CUs, against the 1024 it took before).
+template <class K>
+static unsigned round_blocks(uint64_t work, K kernel) {
+  static std::mutex mtx;
+  static std::map<std::pair<int, const void*>, unsigned> cap_of;
+  int dev = 0;
+  PK_HIP(hipGetDevice(&dev));
+  unsigned cap;
+  {
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = cap_of.find({dev, reinterpret_cast<const void*>(kernel)});
+    if (it == cap_of.end()) {
+      int per_cu = 0, cus = 0;
+      PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kRoundThreads, 0));
+      PK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const uint64_t c = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
+      it = cap_of.emplace(std::make_pair(dev, reinterpret_cast<const void*>(kernel)),
+                          (unsigned)std::min<uint64_t>(c, kMaxRoundBlocks)).first;
+    }
+    cap = it->second;
+  }
   uint64_t b = (work + kRoundThreads - 1) / kRoundThreads;
   if (b < 1) b = 1;
-  if (b > kMaxRoundBlocks) b = kMaxRoundBlocks;
+  if (b > cap) b = cap;
   return (unsigned)b;
 }
 
 void sc_round_dev(fr_t* a, fr_t* b, fr_t* c, fr_t* eq, uint64_t size, const fr_t* fold,
                   fr_t* partials, fr_t* out3, hipStream_t s) {
   const uint64_t h = fold ? size / 4 : size / 2;
-  const unsigned nb = round_blocks(h);
+  const unsigned nb = round_blocks(h, k_sc_round<false>);
   k_sc_round<false><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, fold, partials);
   k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, out3);
   PK_HIP(hipGetLastError());
@@ -868,7 +897,7 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
   // with no rounds (m = 0) it takes its own pass.  The reference mode's
   // claim is rho * sum_g, from the blinding polynomial alone.
   if (mode == 0 && m == 0) {
-    const unsigned nb = round_blocks(size);
+    const unsigned nb = round_blocks(size, k_sc_claim);
     k_sc_claim<<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, size, partials);
     k_sum_partials<<<1, kRoundThreads, 0, s>>>(partials, nb, h3);
   }
@@ -885,7 +914,9 @@ void sc_prove_dev(ScState* st, fr_t* a, fr_t* b, fr_t* c, fr_t* eq, int m, int m
     }
     // two launches per round: fold + map + workgroup sums, then the
     // sum of the partials and the transcript step in one workgroup
-    const unsigned nb = round_blocks(fold ? cur / 4 : cur / 2);
+    const uint64_t pairs = fold ? cur / 4 : cur / 2;
+    const unsigned nb = (idx == 0 && mode == 0) ? round_blocks(pairs, k_sc_round<true>)
+                                                : round_blocks(pairs, k_sc_round<false>);
     if (idx == 0 && mode == 0) {
       k_sc_round<true><<<nb, kRoundThreads, 0, s>>>(a, b, c, eq, cur, fold, partials);
       k_sc_finalize<4><<<1, kFinThreads, 0, s>>>(st, partials, nb, idx);
