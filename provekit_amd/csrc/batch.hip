@@ -49,6 +49,17 @@ __global__ __launch_bounds__(256) void k_coset_scale_batch(const BatchPolyDev* p
   fr_store(p.chunk + i, v);
 }
 
+// A polynomial of one coefficient has no NTT stage: its codeword is that
+// coefficient (coset-scaled by 5^0 = 1 when asked) in every position, as the
+// per-call encoder's replication leaves it.
+__global__ __launch_bounds__(256) void k_replicate_batch(const BatchPolyDev* polys, const uint2* map) {
+  const uint2 m = map[blockIdx.x];
+  const BatchPolyDev& p = polys[m.x];
+  const uint64_t i = (uint64_t)m.y * 256 + threadIdx.x;
+  if (i >= p.N) return;
+  fr_store(p.codeword + i, fr_load(p.chunk ? p.chunk : p.coeffs));
+}
+
 __global__ void k_fill_zero_u32(uint32_t* p, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -220,7 +231,7 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
   auto plan = std::make_unique<BatchPlan>();
   plan->count = (uint32_t)specs.size();
   std::vector<BatchPolyDev> polys(specs.size());
-  std::vector<uint2> coset_map;
+  std::vector<uint2> coset_map, rep_map;
   uint64_t chunk_total = 0, max_n = 0;
   for (size_t i = 0; i < specs.size(); i++) {
     const BatchPolySpec& s = specs[i];
@@ -263,12 +274,15 @@ BatchPlan* batch_plan_create(Device& d, const std::vector<BatchPolySpec>& specs,
     NttShape shape;
     shape.tile_log = tile_log;  // 0: the tuned shape of this length
     nplans[i] = ntt_plan(d, s.codeword, p.N, p.N, s.blowup, src, s.n, shape);
-    PK_REQUIRE(!nplans[i].empty(), PK_ERR_SIZE, "batch: blowup must be below the codeword size");
+    if (nplans[i].empty())  // n = 1: no stage, the codeword is the coefficient replicated
+      for (uint64_t b = 0; b * 256 < p.N; b++) rep_map.push_back(make_uint2((uint32_t)i, (uint32_t)b));
     max_rounds = std::max(max_rounds, nplans[i].size());
   }
   plan->polys = upload_vec(polys, plan->owned);
   plan->coset_map = upload_vec(coset_map, plan->owned);
   plan->coset_blocks = (uint32_t)coset_map.size();
+  plan->rep_map = upload_vec(rep_map, plan->owned);
+  plan->rep_blocks = (uint32_t)rep_map.size();
   // pass rounds, grouped by kernel shape (tile size, butterflies per thread)
   std::vector<NttPassDesc> descs;
   for (size_t r = 0; r < max_rounds; r++) {
@@ -371,6 +385,7 @@ void batch_run(BatchPlan& plan, hipStream_t s) {
   if (plan.coset_blocks)
     k_coset_scale_batch<<<plan.coset_blocks, 256, 0, s>>>(plan.polys, plan.coset_map, plan.coset);
   PK_HIP(hipGetLastError());
+  if (plan.rep_blocks) k_replicate_batch<<<plan.rep_blocks, 256, 0, s>>>(plan.polys, plan.rep_map);
   for (const BatchPlan::Round& rd : plan.rounds) ntt_batch_round_dev(plan.descs, rd.map, rd.blocks, rd.tl, rd.bpt, s);
   k_hash_queue<<<plan.queue_blocks, kQueueBlock, 0, s>>>(plan.polys, plan.tasks, plan.ntasks,
                                                          plan.done, plan.next, plan.err);

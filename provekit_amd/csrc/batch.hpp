@@ -49,6 +49,8 @@ struct BatchPlan {
   BatchPolyDev* polys = nullptr;
   const uint2* coset_map = nullptr;
   uint32_t coset_blocks = 0;
+  const uint2* rep_map = nullptr;  // (polynomial, 256-element block) of single-coefficient codewords
+  uint32_t rep_blocks = 0;
   const fr_t* coset = nullptr;
   NttPassDesc* descs = nullptr;
   std::vector<Round> rounds;

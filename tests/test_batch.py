@@ -59,6 +59,24 @@ def test_batch_leaf_sizes(gpu, oracle, leaf_elems):
     check(gpu, oracle, batch, host, batch.roots())
 
 
+def test_batch_single_coefficient_polynomials(gpu, oracle):
+    # k = 0 has no NTT stage: the codeword is the coefficient in every
+    # position, with and without the coset, beside transforms that have
+    # passes, on plain runs and graph replays; a one-leaf tree is its digest
+    ks = [0, 5, 0, 0, 9, 0]
+    blowups = [2, 4, 8, 4, 2, 2]
+    cosets = [True, False, False, True, True, False]
+    polys, host = make_batch(gpu, ks, blowups, cosets, seed=900)
+    print(f"[case] batch with single-coefficient polys k={ks}", flush=True)
+    batch = gpu.BatchCommit(polys, leaf_elems=2)
+    batch.run()
+    check(gpu, oracle, batch, host, batch.roots())
+    for b in batch.codewords:
+        b.upload(np.zeros(b.nbytes // 8, np.uint64))
+    batch.graph_launch()
+    check(gpu, oracle, batch, host, batch.roots())
+
+
 def test_batch_graph_replays(gpu, oracle):
     ks = [12, 15, 11, 17, 13]
     polys, host = make_batch(gpu, ks, [4, 2, 4, 2, 4], [True] * 5, seed=7)
