@@ -216,6 +216,10 @@ int or_rs_interleaved_encode(const ofr* messages, size_t num_messages, size_t me
                              ofr* out) {
   if (num_messages == 0) return 0;
   size_t total = num_messages * codeword_length;
+  /* the reference indexes result[column * M + row] and slices the masks into
+   * the M x codeword_length vector: past its end it panics (provekit/common/src/ntt.rs:70-80) */
+  if (message_length > codeword_length || message_length * num_messages + masks_len > total) return -1;
+  if (codeword_length == 0) return 0; /* empty messages, empty codeword */
   memset(out, 0, sizeof(ofr) * total);
   for (size_t col = 0; col < message_length; col++)
     for (size_t row = 0; row < num_messages; row++)
@@ -227,6 +231,8 @@ int or_rs_interleaved_encode(const ofr* messages, size_t num_messages, size_t me
   while (codeword_length % coset_size) {
     if (!or_rs_next_order(coset_size + 1, &coset_size)) return -1;
   }
+  /* ntt_nr asserts a power-of-two codeword above 1 (ntt/src/ntt.rs:160-169) */
+  if (codeword_length > 1 && (codeword_length & (codeword_length - 1))) return -1;
   size_t num_cosets = codeword_length / coset_size;
   size_t chunk = coset_size * num_messages;
   for (size_t k = 1; k < num_cosets; k++) memmove(out + k * chunk, out, sizeof(ofr) * chunk);

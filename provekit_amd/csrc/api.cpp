@@ -163,9 +163,10 @@ std::vector<Fr> RSFr::interleaved_encode(const std::vector<const Fr*>& messages,
                                          size_t codeword_length) const {
   if (messages.empty()) return {};
   const size_t M = messages.size();
-  REQ(pk::is_pow2(codeword_length), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
   REQ(M * message_length + masks.size() <= M * codeword_length, pk::PK_ERR_SIZE,
       "interleaved_encode: messages and masks longer than the codeword");
+  if (codeword_length == 0) return {};  // empty messages into an empty codeword
+  REQ(pk::is_pow2(codeword_length), pk::PK_ERR_NOT_POW2, "codeword_size.is_power_of_two()");
   hipStream_t s = pk::thread_stream();
   pk::Device& d = pk::device_current();
   Staged dm(sizeof(fr_t) * M * message_length, s);

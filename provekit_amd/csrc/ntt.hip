@@ -518,6 +518,11 @@ void rs_points_dev(Device& d, uint64_t codeword, const uint64_t* idx, uint64_t n
 void rs_encode_dev(Device& d, const fr_t* msgs, uint64_t M, uint64_t msg_len, const fr_t* masks,
                    uint64_t masks_len, uint64_t codeword, fr_t* out, int coset, hipStream_t s) {
   if (M == 0) return;
+  // every message column and mask lands inside the M x codeword result, or
+  // the reference's indexing panics (provekit/common/src/ntt.rs:70-80)
+  PK_REQUIRE(msg_len <= codeword && msg_len * M + masks_len <= codeword * M, PK_ERR_SIZE,
+             "interleaved_encode: messages and masks longer than the codeword");
+  if (codeword == 0) return;  // empty messages into an empty codeword
   PK_REQUIRE(is_pow2(codeword), PK_ERR_NOT_POW2, "interleaved_encode: codeword length must be a power of two");
   const uint64_t mask_len = masks_len / M;
   uint64_t coset_size = 1;

@@ -150,6 +150,26 @@ def test_rs_interleaved_encode(gpu, oracle, log_msg, log_extra, M, log_mask):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("mlen,M,nmask,codeword,outcome", [
+    (1922, 1, 1, 4, "panic"), (3, 2, 3, 4, "panic"), (3, 2, 2, 4, "ok"), (2, 2, 1, 4, "ok"),
+    (0, 3, 0, 0, "empty"), (1, 1, 0, 0, "panic"), (1, 2, 0, 1, "ok"), (3, 1, 0, 12, "panic")])
+def test_rs_interleaved_encode_bounds(gpu, oracle, mlen, M, nmask, codeword, outcome):
+    # the reference's own bounds (provekit/common/src/ntt.rs:70-80): panics
+    # come back as errors, the rest equal the oracle (pinned on the CPU)
+    msgs = [random_fr(mlen, seed=r, stream="rs-bounds") if mlen else np.zeros((0, 4), np.uint64)
+            for r in range(M)]
+    masks = random_fr(nmask, seed=5, stream="rs-bounds-mask") if nmask else np.zeros((0, 4), np.uint64)
+    say(f"interleaved_encode bounds msg={mlen} M={M} masks={nmask} codeword={codeword}")
+    if outcome == "panic":
+        with pytest.raises(gpu.ProvekitGpuError):
+            gpu.RSFr().interleaved_encode(msgs, masks, codeword)
+        return
+    got = gpu.RSFr().interleaved_encode(msgs, masks, codeword)
+    assert np.array_equal(got, oracle.rs_interleaved_encode(msgs, masks, codeword))
+    if outcome == "empty":
+        assert got.shape == (0, 4)
+
+
 def test_rs_no_masks_blowup(gpu, oracle):
     # the commit shape: one message, no masks, blowup 4 (num_cosets = 4)
     m = random_fr(1 << 14, seed=3, stream="rs-commit")

@@ -296,6 +296,57 @@ def test_rs_interleaved_encode_closed_form(oracle, log_msg, log_extra, M, log_ma
     assert [unmont(x) for x in got] == brute_rs(msgs, masks, codeword)
 
 
+RS_BOUNDS = [
+    # (message length, M, masks, codeword, outcome): the reference writes
+    # result[column * M + row] and the masks into an M x codeword vector,
+    # panicking past its end (provekit/common/src/ntt.rs:70-80); ntt_nr
+    # asserts a power-of-two codeword above 1 (ntt/src/ntt.rs:160-169)
+    (1922, 1, 1, 4, "panic"),    # message longer than the codeword
+    (3, 2, 3, 4, "panic"),       # 6 + 3 entries in 8 slots (masks / M hides the excess)
+    (3, 2, 2, 4, "ok"),          # 6 + 2 entries fill 8 slots
+    (2, 2, 1, 4, "ok"),          # the mask sits past the chunk: the copy overwrites it
+    (0, 3, 0, 0, "empty"),       # empty messages into an empty codeword
+    (1, 1, 0, 0, "panic"),
+    (1, 2, 0, 1, "ok"),          # codeword 1: ntt_nr returns at once
+    (3, 1, 0, 12, "panic"),      # not a power of two
+]
+
+
+@pytest.mark.parametrize("mlen,M,nmask,codeword,outcome", RS_BOUNDS)
+def test_rs_interleaved_encode_bounds(oracle, mlen, M, nmask, codeword, outcome):
+    rng = random.Random(mlen * 7 + M + nmask * 13 + codeword)
+    msgs = [[rng.randrange(P) for _ in range(mlen)] for _ in range(M)]
+    masks = [rng.randrange(P) for _ in range(nmask)]
+    args = ([np.array([mont(x) for x in m], dtype=np.uint64).reshape(-1, 4) for m in msgs],
+            np.array([mont(x) for x in masks], dtype=np.uint64).reshape(-1, 4), codeword)
+    if outcome == "panic":
+        with pytest.raises(ValueError):
+            oracle.rs_interleaved_encode(*args)
+        return
+    got = oracle.rs_interleaved_encode(*args)
+    if outcome == "empty":
+        assert got.shape == (0, 4)
+        return
+    # the reference's steps with Python integers: interleave, masks, the
+    # coset copies, then the transform's closed form on each coset
+    res = [0] * (M * codeword)
+    for c in range(mlen):
+        for r in range(M):
+            res[c * M + r] = msgs[r][c]
+    res[mlen * M: mlen * M + nmask] = masks
+    cs = 1 << max(mlen + nmask // M - 1, 0).bit_length()
+    while codeword % cs:
+        cs *= 2
+    chunk = cs * M
+    for k in range(1, codeword // cs):
+        res[k * chunk:(k + 1) * chunk] = res[:chunk]
+    expect = [unmont(x) for x in got]
+    if codeword == 1:
+        assert expect == res
+    else:
+        assert expect == brute_rs([[res[c * M + r] for c in range(cs)] for r in range(M)], [], codeword)
+
+
 def test_rs_regressions(oracle, kats):
     # provekit/common/proptest-regressions/ntt.txt: elements equal to 2^192
     assert len(kats["ntt_regressions_raw"]) == 2
