@@ -74,6 +74,7 @@ Server::~Server() {
   {
     std::lock_guard<std::mutex> lk(mu_);
     stop_ = true;
+    events_++;
     // jobs never admitted are dropped; running ones finish first
     for (uint64_t id : pending_) {
       Job& j = *jobs_.at(id);
@@ -140,6 +141,7 @@ uint64_t Server::submit(const JobSpec& spec) {
     j->id = id;
     pending_.push_back(id);
     jobs_.emplace(id, std::move(j));
+    events_++;
   }
   cv_.notify_all();
   return id;
@@ -152,6 +154,7 @@ void Server::on_done(void* arg) {
     std::lock_guard<std::mutex> lk(s->mu_);
     j->t_done = Clock::now();
     j->state = Job::Done;  // results are in pinned memory; scheduler finishes bookkeeping
+    s->events_++;
   }
   s->cv_.notify_all();
 }
@@ -242,6 +245,7 @@ void Server::scheduler() {
   (void)hipSetDevice(device_);
   std::unique_lock<std::mutex> lk(mu_);
   for (;;) {
+    const uint64_t seen = events_;  // what this pass accounts for
     // finished jobs: free their slot and transient budget
     for (auto& kv : jobs_) {
       Job& j = *kv.second;
@@ -280,12 +284,15 @@ void Server::scheduler() {
         j.result.error = e.what();
         j.t_done = Clock::now();
         j.state = Job::Done;  // finished (as failed) on the next pass of the loop
+        events_++;
         continue;
       }
       lk.lock();
     }
     if (stop_ && in_flight_ == 0 && pending_.empty()) break;
-    cv_.wait(lk);
+    // a job that completed while launch() ran unlocked has bumped events_:
+    // scan again instead of sleeping past its completion
+    cv_.wait(lk, [&] { return events_ != seen; });
   }
 }
 
@@ -380,6 +387,7 @@ void Server::release(uint64_t id) {
     j = std::move(it->second);
     jobs_.erase(it);
     if (!j->failed) in_use_ -= j->resident_bytes;
+    events_++;  // the freed budget may admit a pending job
   }
   hipStream_t s = thread_stream();
   if (j->codeword) PK_HIP(hipFreeAsync(j->codeword, s));

@@ -67,6 +67,11 @@ class Server {
   std::vector<fr_t*> host_root_;
   std::mutex mu_;
   std::condition_variable cv_;        // scheduler wake-ups
+  // Bumped under mu_ by every change the scheduler acts on (a submit, a job
+  // done, a release, shut-down, a failed launch); the scheduler sleeps only
+  // while it is unchanged since its last scan, so an event that lands while
+  // it is launching (mu_ released) is not lost.
+  uint64_t events_ = 0;
   std::condition_variable done_cv_;   // waiters
   std::deque<uint64_t> pending_;
   std::map<uint64_t, std::unique_ptr<Job>> jobs_;

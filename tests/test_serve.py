@@ -106,6 +106,23 @@ def test_mix_equals_alone(gpu):
             assert np.array_equal(a[key], b[key])
 
 
+def test_lone_small_jobs_complete(gpu):
+    # A job so small that its device work (and the completion callback) can
+    # finish while the scheduler is still inside its launch, with nothing
+    # else queued: the completion must not be lost (the wait would then run
+    # out).  Many fresh servers, one tiny job each.
+    mats, z = make_job(2, 11)
+    roots = set()
+    for i in range(120):
+        srv = gpu.Server(hbm_budget_bytes=1 << 30, slots=1 + i % 4)
+        jid = srv.submit(2, mats, z, seed=5)
+        res = srv.wait(jid, timeout_ms=20000)
+        roots.add(res["root"])
+        srv.release(jid)
+        srv.destroy()
+    assert len(roots) == 1
+
+
 def test_budget_admission(gpu):
     import provekit_amd as pk
     m = 12
