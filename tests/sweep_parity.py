@@ -3,6 +3,7 @@ and shapes for every entry-point family, each case compared with the oracle
 (bit-exact, or the same rejection where the reference panics).
 
     python tests/sweep_parity.py [seconds=180] [seed=12345] [family,family,...]
+    PK_SWEEP_LARGE=1 python tests/sweep_parity.py ...   (larger shapes)
 
 Families: batch, commit, ntt, inverse, sumcheck, hash_many, poseidon2, spmv,
 wire (valid, truncated, trailing bytes, a corrupted interned value),
@@ -24,6 +25,13 @@ from oracle import pyoracle as oracle  # noqa: E402
 BUDGET = float(sys.argv[1]) if len(sys.argv) > 1 else 180.0
 SEED = int(sys.argv[2]) if len(sys.argv) > 2 else 12345
 rng = random.Random(SEED)
+# PK_SWEEP_LARGE=1: the same families at sizes where the multi-pass NTTs,
+# the sumcheck's grid rounds and the chunked SpMV rows take over
+LARGE = os.environ.get("PK_SWEEP_LARGE") == "1"
+
+
+def big(small, large):
+    return large if LARGE else small
 t_end = time.time() + BUDGET
 fails = []
 counts = {}
@@ -40,12 +48,12 @@ def fail(name, kw, msg):
 
 
 def c_batch():
-    count = rng.randint(1, 14)
+    count = rng.randint(1, big(14, 6))
     leaf = rng.choice([2, 4, 8, 16, 32])
     ks, bs, cs = [], [], []
     for _ in range(count):
         while True:
-            k = rng.randint(0, 14)
+            k = rng.randint(0, big(14, 18))
             b = rng.choice([1, 2, 4, 8])
             if (1 << k) * b >= leaf:
                 break
@@ -79,7 +87,7 @@ def c_batch():
 def c_commit():
     leaf = rng.choice([2, 4, 8, 16, 32, 64])
     while True:
-        k = rng.randint(0, 17)
+        k = rng.randint(big(0, 14), big(17, 20))
         b = rng.choice([1, 2, 4, 8])
         if (1 << k) * b >= leaf:
             break
@@ -95,8 +103,8 @@ def c_commit():
 
 
 def c_ntt():
-    log = rng.randint(0, 17)
-    m_log = rng.randint(0, min(3, 20 - log))
+    log = rng.randint(big(0, 16), big(17, 21))
+    m_log = rng.randint(0, min(3, big(20, 22) - log))
     groups = 1 << rng.randint(0, log) if log else 1
     seed = rng.randrange(1 << 30)
     n = 1 << log
@@ -121,7 +129,7 @@ def c_ntt():
 
 
 def c_inverse():
-    log = rng.randint(0, 18)
+    log = rng.randint(big(0, 16), big(18, 21))
     seed = rng.randrange(1 << 30)
     kind = rng.choice(["intt_rn", "intt_nr", "reverse"])
     kw = dict(n=1 << log, kind=kind, seed=seed)
@@ -141,7 +149,7 @@ def c_inverse():
 
 
 def c_sumcheck():
-    m = rng.randint(0, 13)
+    m = rng.randint(big(0, 11), big(13, 18))
     zk = m >= 1 and rng.random() < 0.5
     seed = rng.randrange(1 << 30)
     tseed = rng.randrange(1 << 20)
@@ -193,11 +201,12 @@ def c_poseidon2():
 
 
 def c_spmv():
-    rows = rng.randint(1, 5000)
-    cols = rng.randint(1, 5000)
+    per_max = rng.choice(big([2, 5, 40, 300], [2, 5, 8, 3000]))
+    rows = rng.randint(1, min(big(5000, 1 << 18), 4_000_000 // per_max))
+    cols = rng.randint(1, big(5000, 1 << 18))
     seed = rng.randrange(1 << 30)
     nrng = np.random.default_rng(seed)
-    per = nrng.integers(0, rng.choice([2, 5, 40, 300]), rows)
+    per = nrng.integers(0, per_max, rows)
     rs = np.zeros(rows + 1, np.uint64)
     rs[1:] = np.cumsum(per)
     nnz = int(rs[-1])
@@ -317,8 +326,8 @@ def c_scalar():
 
 
 def c_transpose():
-    rows = rng.randint(1, 3000)
-    cols = rng.randint(1, 3000)
+    rows = rng.randint(1, big(3000, 1 << 17))
+    cols = rng.randint(1, big(3000, 1 << 17))
     seed = rng.randrange(1 << 30)
     nrng = np.random.default_rng(seed)
     per = nrng.integers(0, rng.choice([2, 6, 50]), rows)
@@ -335,7 +344,7 @@ def c_transpose():
 
 
 def c_eq():
-    m = rng.randint(0, 15)
+    m = rng.randint(0, big(15, 20))
     ne = rng.randint(1, 1 << m)
     seed = rng.randrange(1 << 30)
     kw = dict(m=m, num_entries=ne, seed=seed)
@@ -366,7 +375,7 @@ def c_serve():
     blow = rng.choice([2, 4])
     coset = rng.random() < 0.5
     leaf = rng.choice([8, 16])
-    specs = [(rng.randint(1, 11), rng.randrange(1 << 20), rng.randrange(1 << 20)) for _ in range(njobs)]
+    specs = [(rng.randint(1, big(11, 15)), rng.randrange(1 << 20), rng.randrange(1 << 20)) for _ in range(njobs)]
     kw = dict(slots=slots, jobs=specs, blowup=blow, coset=coset, leaf=leaf)
     case("serve", **kw)
     srv = gpu.Server(hbm_budget_bytes=2 << 30, slots=slots)
@@ -405,7 +414,7 @@ def c_serve():
 def c_open():
     leaf = rng.choice([2, 4, 8, 16, 32])
     while True:
-        k = rng.randint(0, 15)
+        k = rng.randint(0, big(15, 19))
         b = rng.choice([1, 2, 4, 8])
         if (1 << k) * b >= leaf:
             break
@@ -446,7 +455,7 @@ def rand_csr(nrng, rows, cols, per_max):
 
 
 def c_interned():
-    rows, cols = rng.randint(1, 4000), rng.randint(1, 4000)
+    rows, cols = rng.randint(1, big(4000, 1 << 17)), rng.randint(1, big(4000, 1 << 17))
     ni = rng.randint(1, 300)
     seed = rng.randrange(1 << 30)
     nrng = np.random.default_rng(seed)
@@ -470,8 +479,8 @@ def c_interned():
 
 
 def c_transposed_eq():
-    n = rng.randint(1, 3000)
-    cols = rng.randint(1, 3000)
+    n = rng.randint(1, big(3000, 1 << 16))
+    cols = rng.randint(1, big(3000, 1 << 16))
     seed = rng.randrange(1 << 30)
     nrng = np.random.default_rng(seed)
     mats = []
