@@ -8,7 +8,7 @@ and shapes for every entry-point family, each case compared with the oracle
 Families: batch, commit, ntt, inverse, sumcheck, hash_many, poseidon2, spmv,
 wire (valid, truncated, trailing bytes, a corrupted interned value),
 whir_commit, rs_encode, scalar, transpose, eq, pow, serve (several jobs per
-server, openings), open, interned, transposed_eq, p2hash.  Time-bounded;
+server, openings), open, interned, transposed_eq, p2hash, bounds.  Time-bounded;
 prints each case before it runs and exits non-zero on any mismatch."""
 import os
 import random
@@ -518,9 +518,32 @@ def c_p2hash():
             return
 
 
+def c_bounds():
+    rows = rng.randint(1, big(3000, 1 << 16))
+    cols = rng.randint(1, big(3000, 1 << 16))
+    seed = rng.randrange(1 << 30)
+    nrng = np.random.default_rng(seed)
+    mats = []
+    for k in range(2):
+        rs, cs, nnz = rand_csr(nrng, rows, cols, rng.choice([2, 5, 20]))
+        vs = random_fr(nnz, seed=seed + k, stream="fuzz-wb") if nnz else np.zeros((0, 4), np.uint64)
+        mats.append((rs, cs, vs))
+    w = random_fr(cols, seed=seed, stream="fuzz-ww")
+    kw = dict(rows=rows, cols=cols, seed=seed)
+    case("bounds", **kw)
+    a, b, c = gpu.calculate_witness_bounds(*[gpu.SparseMatrix(rows, cols, *m) for m in mats], w)
+    n = 1 << max((rows - 1).bit_length(), 0)
+    ea, eb = (oracle.sparse_mul(rows, cols, *m, w) for m in mats)
+    ec = np.stack([oracle.fr_mul(ea[i], eb[i]) for i in range(rows)])
+    ok = a.shape[0] == n and all(np.array_equal(x[:rows], e) and not x[rows:].any()
+                                 for x, e in ((a, ea), (b, eb), (c, ec)))
+    if not ok:
+        fail("bounds", kw, "vectors")
+
+
 CASES = [c_batch, c_commit, c_ntt, c_inverse, c_sumcheck, c_hash_many, c_poseidon2, c_spmv,
          c_wire, c_whir_commit, c_rs_encode, c_scalar, c_transpose, c_eq, c_pow,
-         c_serve, c_open, c_interned, c_transposed_eq, c_p2hash]
+         c_serve, c_open, c_interned, c_transposed_eq, c_p2hash, c_bounds]
 if len(sys.argv) > 3:
     CASES = [c for c in CASES if c.__name__[2:] in sys.argv[3].split(",")]
 gpu.init(0)
