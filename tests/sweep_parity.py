@@ -4,6 +4,7 @@ and shapes for every entry-point family, each case compared with the oracle
 
     python tests/sweep_parity.py [seconds=180] [seed=12345] [family,family,...]
     PK_SWEEP_LARGE=1 python tests/sweep_parity.py ...   (larger shapes)
+    PK_SWEEP_THREADS=8 python tests/sweep_parity.py ... (host threads at once)
 
 Families: batch, commit, ntt, inverse, sumcheck, hash_many, poseidon2, spmv,
 wire (valid, truncated, trailing bytes, a corrupted interned value),
@@ -13,6 +14,7 @@ prints each case before it runs and exits non-zero on any mismatch."""
 import os
 import random
 import sys
+import threading
 import time
 
 import numpy as np
@@ -24,7 +26,20 @@ from oracle import pyoracle as oracle  # noqa: E402
 
 BUDGET = float(sys.argv[1]) if len(sys.argv) > 1 else 180.0
 SEED = int(sys.argv[2]) if len(sys.argv) > 2 else 12345
-rng = random.Random(SEED)
+THREADS = int(os.environ.get("PK_SWEEP_THREADS", "1"))
+_local = threading.local()
+
+
+class _ThreadRng:
+    """One random.Random per host thread (PK_SWEEP_THREADS workers)."""
+
+    def __getattr__(self, name):
+        return getattr(_local.r, name)
+
+
+rng = _ThreadRng()
+_local.r = random.Random(SEED)
+_lock = threading.Lock()
 # PK_SWEEP_LARGE=1: the same families at sizes where the multi-pass NTTs,
 # the sumcheck's grid rounds and the chunked SpMV rows take over
 LARGE = os.environ.get("PK_SWEEP_LARGE") == "1"
@@ -38,13 +53,15 @@ counts = {}
 
 
 def case(name, **kw):
-    print(f"[case] {name} {kw}", flush=True)
-    counts[name] = counts.get(name, 0) + 1
+    with _lock:
+        print(f"[case] {name} {kw}", flush=True)
+        counts[name] = counts.get(name, 0) + 1
 
 
 def fail(name, kw, msg):
-    print(f"[FAIL] {name} {kw}: {msg}", flush=True)
-    fails.append((name, kw, msg))
+    with _lock:
+        print(f"[FAIL] {name} {kw}: {msg}", flush=True)
+        fails.append((name, kw, msg))
 
 
 def c_batch():
@@ -548,16 +565,34 @@ if len(sys.argv) > 3:
     CASES = [c for c in CASES if c.__name__[2:] in sys.argv[3].split(",")]
 gpu.init(0)
 oracle.lib()
-i = 0
-while time.time() < t_end:
-    fn = CASES[i % len(CASES)] if i < len(CASES) else rng.choice(CASES)
-    i += 1
-    try:
-        fn()
-    except gpu.ProvekitGpuError as e:
-        print(f"[error] {fn.__name__}: {e}", flush=True)
-        fails.append((fn.__name__, {}, str(e)))
-print({"cases": counts, "total": i, "failures": len(fails)}, flush=True)
+total = [0]
+
+
+def worker(t):
+    # PK_SWEEP_THREADS > 1: host threads call the library at once (each has
+    # its own stream), as rayon workers calling the plugin would
+    _local.r = random.Random(SEED + 7919 * t)
+    i = 0
+    while time.time() < t_end:
+        fn = CASES[(i + t) % len(CASES)] if i < len(CASES) else rng.choice(CASES)
+        i += 1
+        try:
+            fn()
+        except gpu.ProvekitGpuError as e:
+            fail(fn.__name__, {}, str(e))
+    with _lock:
+        total[0] += i
+
+
+if THREADS == 1:
+    worker(0)
+else:
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(THREADS)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+print({"cases": counts, "total": total[0], "threads": THREADS, "failures": len(fails)}, flush=True)
 for f in fails:
     print("FAILURE", f, flush=True)
 sys.exit(1 if fails else 0)
