@@ -9,7 +9,8 @@ and shapes for every entry-point family, each case compared with the oracle
 Families: batch, commit, ntt, inverse, sumcheck, hash_many, poseidon2, spmv,
 wire (valid, truncated, trailing bytes, a corrupted interned value),
 whir_commit, rs_encode, scalar, transpose, eq, pow, serve (several jobs per
-server, openings), open, interned, transposed_eq, p2hash, bounds.  Time-bounded;
+server, openings), open, interned, transposed_eq, p2hash, bounds, device (chains of _device calls on
+a caller's stream).  Time-bounded;
 prints each case before it runs and exits non-zero on any mismatch."""
 import os
 import random
@@ -558,9 +559,83 @@ def c_bounds():
         fail("bounds", kw, "vectors")
 
 
+_hip = None
+
+
+def user_stream():
+    """A non-blocking stream of this thread, made with the HIP runtime (as a
+    caller keeping data resident would), and a sync for it."""
+    global _hip
+    import ctypes
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    if getattr(_local, "stream", None) is None:
+        st = ctypes.c_void_p()
+        assert _hip.hipStreamCreateWithFlags(ctypes.byref(st), 1) == 0
+        _local.stream = st.value
+    return _local.stream, lambda: _hip.hipStreamSynchronize(_local.stream)
+
+
+def c_device():
+    """Device entry points queued back to back on a caller's non-blocking
+    stream, one host sync at the end."""
+    s, sync = user_stream()
+    log = rng.randint(0, big(14, 20))
+    n = 1 << log
+    ops = []
+    for _ in range(rng.randint(1, 4)):
+        op = rng.choice(["ntt", "intt_rn", "intt_nr", "reverse"])
+        arg = 1 << rng.randint(0, log) if op in ("ntt", "reverse") else 0
+        ops.append((op, arg))
+    leaf = rng.choice([2, 4, 8, 16])
+    blow = rng.choice([1, 2, 4])
+    coset = rng.random() < 0.5
+    seed = rng.randrange(1 << 30)
+    kw = dict(n=n, ops=ops, leaf=leaf, blowup=blow, coset=coset, seed=seed)
+    case("device", **kw)
+    x = random_fr(n, seed=seed, stream="fuzz-dev")
+    d = gpu.DeviceBuffer.from_array(x)
+    want = x.copy()
+    for op, arg in ops:
+        if op == "ntt":
+            gpu.ntt_nr_device(d, n, n, arg, stream=s)
+            oracle.ntt_nr(want, n, arg)
+        elif op == "intt_rn":
+            gpu.intt_rn_device(d, n, stream=s)
+            oracle.intt_rn(want)
+        elif op == "intt_nr":
+            gpu.intt_nr_device(d, n, stream=s)
+            oracle.intt_nr(want)
+        else:
+            gpu.reverse_order_device(d, n, arg, stream=s)
+            oracle.reverse_order(want, arg)
+    # then commit the result from the device and rehash its codeword
+    N = n * blow
+    if N >= leaf:
+        leaves = N // leaf
+        dcw, dn = gpu.DeviceBuffer(32 * N), gpu.DeviceBuffer(32 * (2 * leaves - 1))
+        gpu.commit_device(d, n, blow, coset, leaf, dcw, dn, stream=s)
+        dh, dt = gpu.DeviceBuffer(32 * leaves), gpu.DeviceBuffer(32 * (2 * leaves - 1))
+        gpu.hash_many_device(32 * leaf, dcw, leaves, dh, stream=s)
+        gpu.merkle_device(dh, leaves, dt, stream=s)
+    assert sync() == 0
+    if not np.array_equal(d.download(shape=(n, 4)), want):
+        fail("device", kw, "transform chain")
+        return
+    if N >= leaf:
+        root, cw, nodes = oracle.commit(want, blow, coset, leaf)
+        got_nodes = dn.download(np.uint8).reshape(-1, 32)
+        got_tree = dt.download(np.uint8).reshape(-1, 32)
+        if not (np.array_equal(dcw.download(shape=(N, 4)), cw) and np.array_equal(got_nodes, nodes)
+                and np.array_equal(got_tree, nodes)):
+            fail("device", kw, "commit / rehash")
+
+
 CASES = [c_batch, c_commit, c_ntt, c_inverse, c_sumcheck, c_hash_many, c_poseidon2, c_spmv,
          c_wire, c_whir_commit, c_rs_encode, c_scalar, c_transpose, c_eq, c_pow,
-         c_serve, c_open, c_interned, c_transposed_eq, c_p2hash, c_bounds]
+         c_serve, c_open, c_interned, c_transposed_eq, c_p2hash, c_bounds, c_device]
 if len(sys.argv) > 3:
     CASES = [c for c in CASES if c.__name__[2:] in sys.argv[3].split(",")]
 gpu.init(0)
