@@ -65,16 +65,20 @@ static size_t g_roots_order = 0;
  * workload_size.next_power_of_two(). */
 static const ofr* roots_for(size_t codeword_size) {
   size_t want = codeword_size < WORKLOAD_SIZE ? WORKLOAD_SIZE : codeword_size;
+  const ofr* cur;
 #pragma omp critical(or_roots)
   {
     if (g_roots_order < want) {
-      free(g_roots);
-      g_roots = (ofr*)malloc(sizeof(ofr) * (want / 2));
-      or_roots_table(want, g_roots);
+      /* the smaller table is kept, not freed: a transform on another thread
+       * may still be reading it (the reference's readers hold its lock) */
+      ofr* t = (ofr*)malloc(sizeof(ofr) * (want / 2));
+      or_roots_table(want, t);
+      g_roots = t;
       g_roots_order = want;
     }
+    cur = g_roots;
   }
-  return g_roots;
+  return cur;
 }
 
 static inline void butterfly(ofr* e, ofr* o, ofr w) {

@@ -296,6 +296,54 @@ def test_rs_interleaved_encode_closed_form(oracle, log_msg, log_extra, M, log_ma
     assert [unmont(x) for x in got] == brute_rs(msgs, masks, codeword)
 
 
+_CONCURRENT = r"""
+import hashlib, sys, threading
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+from oracle import pyoracle as o
+from provekit_amd.inputs import random_fr
+o.lib()
+o.set_threads(2)
+sizes = [1 << k for k in range(10, 18)]
+out = {}
+def work(t):
+    for k in sizes[t % 2::2] + sizes[(t + 1) % 2::2]:
+        v = random_fr(k, seed=k, stream="concurrent")
+        o.ntt_nr(v, k, 1)
+        out[(t, k)] = hashlib.sha256(v.tobytes()).hexdigest()
+ths = [threading.Thread(target=work, args=(t,)) for t in range(6)]
+for th in ths: th.start()
+for th in ths: th.join()
+for (t, k), h in sorted(out.items()):
+    print(t, k, h)
+"""
+
+
+def test_oracle_ntt_concurrent_table_growth(oracle):
+    # transforms on several threads while the process-global twiddle table
+    # grows: each equals the single-threaded result (a table replaced while
+    # another thread read it would corrupt that thread's transform)
+    import hashlib
+    import os
+    import subprocess
+    import sys
+    from provekit_amd.inputs import random_fr
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _CONCURRENT, root], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    want = {}
+    for k in [1 << j for j in range(10, 18)]:
+        v = random_fr(k, seed=k, stream="concurrent")
+        oracle.ntt_nr(v, k, 1)
+        want[k] = hashlib.sha256(v.tobytes()).hexdigest()
+    lines = r.stdout.split("\n")
+    got = [ln.split() for ln in lines if ln.strip()]
+    assert len(got) == 6 * 8
+    for t, k, h in got:
+        assert h == want[int(k)], (t, k)
+
+
 RS_BOUNDS = [
     # (message length, M, masks, codeword, outcome): the reference writes
     # result[column * M + row] and the masks into an M x codeword vector,
