@@ -21,9 +21,18 @@ namespace {
 
 thread_local std::string g_last_error;
 
-template <class F>
-int guarded(F&& f) {
+// Entry points hold the capture lock shared (see capture_mutex); the ones
+// that capture a graph take it exclusively instead, through guarded_capture.
+// Entry points that only wait on the server's own condition variables take
+// no lock (guarded_nolock), so a capture is not held up by a long wait.
+enum class Lock { Shared, Exclusive, None };
+template <Lock L, class F>
+int guarded_impl(F&& f) {
   try {
+    std::shared_lock<std::shared_mutex> shared(pk::capture_mutex(), std::defer_lock);
+    std::unique_lock<std::shared_mutex> excl(pk::capture_mutex(), std::defer_lock);
+    if (L == Lock::Exclusive) excl.lock();
+    if (L == Lock::Shared) shared.lock();
     f();
     g_last_error.clear();
     return PKGPU_OK;
@@ -40,6 +49,18 @@ int guarded(F&& f) {
     g_last_error = "unknown failure";
     return PKGPU_ERR_INTERNAL;
   }
+}
+template <class F>
+int guarded(F&& f) {
+  return guarded_impl<Lock::Shared>(std::forward<F>(f));
+}
+template <class F>
+int guarded_capture(F&& f) {
+  return guarded_impl<Lock::Exclusive>(std::forward<F>(f));
+}
+template <class F>
+int guarded_nolock(F&& f) {
+  return guarded_impl<Lock::None>(std::forward<F>(f));
 }
 
 hipStream_t stream_or_thread(void* s) {
@@ -659,7 +680,7 @@ int pkgpu_poseidon2_merkle_device(const uint64_t* leaves, size_t n, uint64_t* no
 }
 int pkgpu_poseidon2_merkle_graph(const uint64_t* leaves, size_t n, uint64_t* nodes,
                                  void** graph) {
-  return guarded([&] {
+  return guarded_capture([&] {
     REQ(leaves && nodes && graph, pk::PK_ERR_ARG, "poseidon2_merkle_graph: null pointer");
     REQ(pk::is_pow2(n), pk::PK_ERR_NOT_POW2, "merkle: leaf count must be a power of two");
     *graph = pk::p2_merkle_graph(pk::device_current(), reinterpret_cast<const fr_t*>(leaves), n,
@@ -1288,7 +1309,7 @@ int pkgpu_batch_run(void* plan, void* stream) {
   });
 }
 int pkgpu_batch_graph_launch(void* plan, void* stream) {
-  return guarded([&] {
+  return guarded_capture([&] {
     REQ(plan, pk::PK_ERR_STATE, "batch: null plan");
     hipGraphExec_t g = pk::batch_graph(*static_cast<pk::BatchPlan*>(plan));
     PK_HIP(hipGraphLaunch(g, stream_or_thread(stream)));
@@ -1351,7 +1372,7 @@ int pkgpu_server_submit(void* server, const pkgpu_job* job, uint64_t* job_id) {
 }
 int pkgpu_server_wait(void* server, uint64_t job_id, double timeout_ms, pkgpu_job_result* result,
                       uint64_t* coeffs, uint64_t* alphas, uint64_t* r) {
-  return guarded([&] {
+  return guarded_nolock([&] {
     REQ(server && result, pk::PK_ERR_ARG, "server: null pointer");
     pk::JobResult jr = static_cast<pk::Server*>(server)->wait(job_id, timeout_ms);
     memcpy(result->root, &jr.root, 32);
@@ -1367,7 +1388,7 @@ int pkgpu_server_wait(void* server, uint64_t job_id, double timeout_ms, pkgpu_jo
   });
 }
 int pkgpu_server_poll(void* server, uint64_t job_id, int* done) {
-  return guarded([&] {
+  return guarded_nolock([&] {
     REQ(server && done, pk::PK_ERR_ARG, "server: null pointer");
     *done = static_cast<pk::Server*>(server)->poll(job_id) ? 1 : 0;
   });

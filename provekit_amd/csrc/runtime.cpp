@@ -77,6 +77,11 @@ struct ThreadStreams {
 };
 }  // namespace
 
+std::shared_mutex& capture_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
+
 hipStream_t thread_stream() {
   thread_local ThreadStreams ts;
   int id = 0;

@@ -100,6 +100,15 @@ struct Scratch {
 // The calling thread's work stream on the current device (default priority).
 hipStream_t thread_stream();
 
+// HIP invalidates a stream capture when another thread makes a synchronous
+// call meanwhile (a hipMemcpy on the legacy stream fails with "would make
+// the legacy stream depend on a capturing blocking stream", and launches on
+// other streams then fail too), even for a thread-local capture on a
+// non-blocking stream.  Every entry point holds this lock shared while it
+// makes HIP calls; the entry points that capture a graph hold it exclusively
+// for the capture, which only enqueues.
+std::shared_mutex& capture_mutex();
+
 // Host-to-device copy on stream s from pageable host memory.  Copies of 32 MiB
 // and more go through pinned staging buffers filled by several host threads
 // while the previous chunk is in flight; smaller ones are a plain async copy.

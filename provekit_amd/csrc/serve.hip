@@ -275,6 +275,7 @@ void Server::scheduler() {
       j.t_start = Clock::now();
       lk.unlock();
       try {
+        std::shared_lock<std::shared_mutex> cap(capture_mutex());  // no capture meanwhile
         launch(j);
       } catch (const std::exception& e) {
         const Error* pe = dynamic_cast<const Error*>(&e);
